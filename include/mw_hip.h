@@ -1,0 +1,151 @@
+/*
+ * mw_hip.h -- C ABI of the MI355X (gfx950) separation likelihood library
+ * (libmw_separation_hip.so).
+ *
+ * Two layers:
+ *
+ * 1. Drop-in entry points with the reference evaluator's signatures and side
+ *    effects.  A client selects them where it selects cpu_evaluate today
+ *    (src/milkyway_client_app.c:263-274) and calls them through the same
+ *    EvalFunc pointer (milkyway_separation/cpu/simple_evaluator.h:26).
+ *
+ * 2. A reentrant context API for batched evaluation and serving: a context
+ *    keeps one wedge's tables and star list resident on the current GPU and
+ *    evaluates many parameter vectors per call.  It returns status codes and
+ *    never exits.
+ *
+ * Limits of this build: at most MW_MAX_STREAMS streams per model and at most
+ * MW_MAX_CONVOLVE convolution points.
+ */
+#ifndef MW_HIP_H
+#define MW_HIP_H
+
+#include <stddef.h>
+
+#include "mw_separation_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MW_MAX_STREAMS 4
+#define MW_MAX_CONVOLVE 1024
+
+/* Status codes */
+#define MW_OK 0
+#define MW_E_INVALID 1      /* inputs the evaluator cannot handle */
+#define MW_E_HIP 2          /* a HIP runtime call failed */
+#define MW_E_NOMEM 3        /* device or host allocation failed */
+#define MW_E_NODEVICE 4     /* no GPU visible */
+
+/* Precision modes */
+#define MW_PREC_FP64 0      /* FP64 throughout */
+#define MW_PREC_MIXED 1     /* FP32 sqrt/reciprocal/exp, FP64 positions, distances, sums, logs */
+
+/* ------------------------------------------------------------------------
+ * Drop-in entry points
+ * ------------------------------------------------------------------------ */
+
+/* Replaces cpu_evaluate (milkyway_separation/cpu/evaluation_optimized.h:59-62,
+ * evaluation_optimized.c:756-784).  Maps the parameter vector into ap
+ * (set_astronomy_parameters, parameters.c:332-363), resets es, computes the
+ * integrals and the star pass with the same side effects on es, and returns
+ * es->prob_sum / (number_stars - bad_jacobians); the client subtracts 3.0.
+ * On failure it prints "APP: error calculating ..." and exits with the error
+ * code, as the reference does through mw_finish. */
+double mw_hip_evaluate(double* parameters, ASTRONOMY_PARAMETERS* ap, EVALUATION_STATE* es,
+                       STAR_POINTS* sp);
+
+/* Replaces calculate_integrals (evaluation_optimized.h:49, .c:564-611).
+ * Honours es->current_integral and each area's current_calculation /
+ * max_calculation (partial and resumed ranges), adds to the areas' existing
+ * partial integrals and writes es->background_integral and
+ * es->stream_integrals as region 0 minus the cut regions.  Returns 0, or a
+ * status code (the reference always returns 0 and exits on failure). */
+int mw_hip_calculate_integrals(const ASTRONOMY_PARAMETERS* ap, EVALUATION_STATE* es,
+                               const STAR_POINTS* sp);
+
+/* Replaces calculate_likelihood (evaluation_optimized.h:50, .c:613-753).
+ * Stars from es->current_star_point on; writes es->prob_sum, es->num_zero and
+ * es->current_star_point.  Returns 0 or a status code. */
+int mw_hip_calculate_likelihood(const ASTRONOMY_PARAMETERS* ap, EVALUATION_STATE* es,
+                                const STAR_POINTS* sp);
+
+/* Precision of the drop-in entry points (default MW_PREC_FP64, or the
+ * environment variable MW_HIP_PRECISION=mixed).  No reference counterpart:
+ * the reference picks precision at build time (DOUBLE_PRECISION,
+ * include/config.h.in:30). */
+int mw_hip_set_precision(int precision);
+
+/* ------------------------------------------------------------------------
+ * Context API (no reference counterpart; the reference's nearest equivalent
+ * is the per-rank integral_f / likelihood_f plugin pair of
+ * milkyway_separation/cpu/astronomy_worker.c:35-124)
+ * ------------------------------------------------------------------------ */
+
+typedef struct mw_ctx mw_ctx;
+
+typedef struct {
+    double likelihood;                 /* prob_sum / number_stars, without the -3.0 */
+    double prob_sum;                   /* sum of log10 P, -238 per zero star */
+    int num_zero;                      /* stars with P == 0 */
+    int number_streams;
+    double background_integral;        /* region 0 minus cut regions */
+    double stream_integrals[MW_MAX_STREAMS];
+    double bg_only_likelihood;         /* mean background-only log10 (no -3.0) */
+    double stream_only_likelihood[MW_MAX_STREAMS];
+} mw_result;
+
+typedef struct {
+    double integral_ms;                /* integral kernel(s) of the last mw_ctx_eval */
+    double star_ms;                    /* star kernel(s) */
+    double total_ms;                   /* first to last kernel, including reductions */
+    long integral_blocks, star_blocks; /* workgroups launched */
+} mw_timing;
+
+/* Flags for mw_ctx_create */
+#define MW_CTX_TIMING 1                /* record HIP events around every kernel */
+
+/* Builds the wedge described by ap (regions, convolve, wedge, sgr_coordinates,
+ * aux_bg_profile, number of streams, optimize masks) and the star list on the
+ * current device.  ap's parameter values are the defaults that parameter
+ * vectors override.  sp may be NULL (integrals only). */
+int mw_ctx_create(const ASTRONOMY_PARAMETERS* ap, const STAR_POINTS* sp, int flags,
+                  mw_ctx** out);
+
+/* Number of doubles in one parameter vector (get_optimized_parameter_count,
+ * parameters.c:307-330). */
+int mw_ctx_parameter_count(const mw_ctx* ctx);
+
+/* Evaluates n_vec parameter vectors (params: n_vec x mw_ctx_parameter_count
+ * doubles, laid out as set_astronomy_parameters reads them) in one batch.
+ * results: n_vec entries.  log10p: NULL or n_vec x number_stars doubles that
+ * receive every star's log10 P (-238 where P == 0). */
+int mw_ctx_eval(mw_ctx* ctx, const double* params, int n_vec, int precision,
+                mw_result* results, double* log10p);
+
+/* Region-level integrals of the last mw_ctx_eval: out[v][g][1 + MW_MAX_STREAMS]
+ * for vector v and region g (background first).  n is the capacity in doubles. */
+int mw_ctx_region_integrals(const mw_ctx* ctx, double* out, size_t n);
+
+int mw_ctx_last_timing(const mw_ctx* ctx, mw_timing* out);
+
+/* Device bytes the context holds resident. */
+size_t mw_ctx_resident_bytes(const mw_ctx* ctx);
+
+void mw_ctx_destroy(mw_ctx* ctx);
+
+/* Message of the last failure on this thread ("" if none). */
+const char* mw_last_error(void);
+
+/* Number of visible GPUs (0 when none). */
+int mw_device_count(void);
+
+/* Waits for all work of the calling thread's current device. */
+int mw_device_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MW_HIP_H */

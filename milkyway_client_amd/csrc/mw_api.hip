@@ -1,0 +1,845 @@
+// mw_api.hip -- C ABI of the separation library: resident wedge contexts,
+// batched evaluation, and the drop-in replacements of the reference
+// evaluator (cpu_evaluate, calculate_integrals, calculate_likelihood).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "mw_hip.h"
+#include "mw_internal.h"
+
+namespace mw {
+hipError_t launch_integral(int S, int bg, bool mixed, const RegionLaunch* d_items, int n_items,
+                           int n_blocks, const EvalConst* d_consts, double* d_partials,
+                           hipStream_t stream);
+hipError_t launch_star(int S, int bg, bool mixed, const StarLaunch* d_items, int n_items,
+                       int n_blocks, const EvalConst* d_consts, const double* d_ratio,
+                       const double* d_kw, double* d_partials, double* d_log10p, long stride,
+                       hipStream_t stream);
+hipError_t launch_reduce(bool star, const void* d_items, int n_items, const double* d_partials,
+                         double* d_rows, hipStream_t stream);
+hipError_t launch_finalize(const int* d_first, const int* d_count, const double* d_rows,
+                           EvalConst* d_consts, int n_slots, hipStream_t stream);
+int tile_pairs();
+int integral_comps();
+int star_comps();
+}  // namespace mw
+
+using namespace mw;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg)
+{
+    g_error = msg;
+    return code;
+}
+
+#define MW_HIP_CHECK(expr)                                                                  \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(MW_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+// Device buffer that grows on demand.
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    int reserve(size_t n)
+    {
+        if (n <= cap) return MW_OK;
+        release();
+        if (hipMalloc(&p, sizeof(T) * std::max<size_t>(n, 1)) != hipSuccess) {
+            p = nullptr;
+            return fail(MW_E_NOMEM, "hipMalloc failed");
+        }
+        cap = n;
+        return MW_OK;
+    }
+};
+
+// Pinned host buffer that grows on demand.
+template <typename T>
+struct HostBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    ~HostBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    int reserve(size_t n)
+    {
+        if (n <= cap) return MW_OK;
+        release();
+        if (hipHostMalloc((void**)&p, sizeof(T) * std::max<size_t>(n, 1), hipHostMallocDefault) != hipSuccess) {
+            p = nullptr;
+            return fail(MW_E_NOMEM, "hipHostMalloc failed");
+        }
+        cap = n;
+        return MW_OK;
+    }
+};
+
+// Geometry of one region, as INTEGRAL_AREA holds it.
+struct RegionGeom {
+    int r_steps, mu_steps, nu_steps;
+    double r_min, r_max, mu_min, mu_max, nu_min, nu_max;
+    double r_step_size, mu_step_size, nu_step_size;
+    bool operator==(const RegionGeom& o) const { return std::memcmp(this, &o, sizeof(*this)) == 0; }
+};
+
+RegionGeom geom_of(const INTEGRAL_AREA& a)
+{
+    RegionGeom g;
+    std::memset(&g, 0, sizeof(g));
+    g.r_steps = a.r_steps; g.mu_steps = a.mu_steps; g.nu_steps = a.nu_steps;
+    g.r_min = a.r_min; g.r_max = a.r_max; g.mu_min = a.mu_min; g.mu_max = a.mu_max;
+    g.nu_min = a.nu_min; g.nu_max = a.nu_max;
+    g.r_step_size = a.r_step_size; g.mu_step_size = a.mu_step_size; g.nu_step_size = a.nu_step_size;
+    return g;
+}
+
+// INTEGRAL -> INTEGRAL_AREA geometry as initialize_integral_area computes it
+// (evaluation_state.c:97-127).
+RegionGeom geom_of(const INTEGRAL& in)
+{
+    INTEGRAL_AREA a;
+    std::memset(&a, 0, sizeof(a));
+    a.mu_min = in.mu_min; a.mu_max = in.mu_max; a.mu_steps = in.mu_steps;
+    a.nu_min = in.nu_min; a.nu_max = in.nu_max; a.nu_steps = in.nu_steps;
+    a.r_min = in.r_min; a.r_max = in.r_max; a.r_steps = in.r_steps;
+    a.mu_step_size = (a.mu_max - a.mu_min) / a.mu_steps;
+    a.nu_step_size = (a.nu_max - a.nu_min) / a.nu_steps;
+    a.r_step_size = (a.r_max - a.r_min) / a.r_steps;
+    return geom_of(a);
+}
+
+INTEGRAL_AREA area_of(const RegionGeom& g)
+{
+    INTEGRAL_AREA a;
+    std::memset(&a, 0, sizeof(a));
+    a.r_steps = g.r_steps; a.mu_steps = g.mu_steps; a.nu_steps = g.nu_steps;
+    a.r_min = g.r_min; a.r_max = g.r_max; a.mu_min = g.mu_min; a.mu_max = g.mu_max;
+    a.nu_min = g.nu_min; a.nu_max = g.nu_max;
+    a.r_step_size = g.r_step_size; a.mu_step_size = g.mu_step_size; a.nu_step_size = g.nu_step_size;
+    return a;
+}
+
+// Deep copy of the model part of ASTRONOMY_PARAMETERS (values and optimize
+// masks) that parameter vectors are written into.
+struct ModelTemplate {
+    ASTRONOMY_PARAMETERS ap{};
+    std::vector<double> bg;
+    std::vector<int> bg_opt;
+    std::vector<double> sw;
+    std::vector<int> sw_opt;
+    std::vector<std::vector<double>> sp;
+    std::vector<std::vector<int>> sp_opt;
+    std::vector<double*> sp_rows;
+    std::vector<int*> sp_opt_rows;
+
+    void load(const ASTRONOMY_PARAMETERS& src)
+    {
+        ap = src;
+        const int nb = src.number_background_parameters, S = src.number_streams;
+        const int nsp = src.number_stream_parameters;
+        bg.assign(src.background_parameters, src.background_parameters + nb);
+        bg_opt.assign(src.background_optimize, src.background_optimize + nb);
+        sw.assign(src.stream_weights, src.stream_weights + S);
+        sw_opt.assign(src.stream_weight_optimize, src.stream_weight_optimize + S);
+        sp.resize(S);
+        sp_opt.resize(S);
+        sp_rows.resize(S);
+        sp_opt_rows.resize(S);
+        for (int s = 0; s < S; s++) {
+            sp[s].assign(src.stream_parameters[s], src.stream_parameters[s] + nsp);
+            sp_opt[s].assign(src.stream_optimize[s], src.stream_optimize[s] + nsp);
+            sp_rows[s] = sp[s].data();
+            sp_opt_rows[s] = sp_opt[s].data();
+        }
+        ap.background_parameters = bg.data();
+        ap.background_optimize = bg_opt.data();
+        ap.stream_weights = sw.data();
+        ap.stream_weight_optimize = sw_opt.data();
+        ap.stream_parameters = sp_rows.data();
+        ap.stream_optimize = sp_opt_rows.data();
+        ap.background_step = ap.background_min = ap.background_max = nullptr;
+        ap.stream_weight_step = ap.stream_weight_min = ap.stream_weight_max = nullptr;
+        ap.stream_step = ap.stream_min = ap.stream_max = nullptr;
+        ap.integral = nullptr;
+    }
+};
+
+// get_optimized_parameter_count (parameters.c:307-330)
+int optimized_count(const ASTRONOMY_PARAMETERS& ap)
+{
+    int n = 0;
+    for (int i = 0; i < ap.number_background_parameters; i++) n += ap.background_optimize[i] ? 1 : 0;
+    for (int s = 0; s < ap.number_streams; s++) {
+        n += ap.stream_weight_optimize[s] ? 1 : 0;
+        for (int j = 0; j < ap.number_stream_parameters; j++) n += ap.stream_optimize[s][j] ? 1 : 0;
+    }
+    return n;
+}
+
+// set_astronomy_parameters (parameters.c:332-363)
+void apply_vector(ASTRONOMY_PARAMETERS& ap, const double* v)
+{
+    int k = 0;
+    for (int i = 0; i < ap.number_background_parameters; i++)
+        if (ap.background_optimize[i]) ap.background_parameters[i] = v[k++];
+    for (int s = 0; s < ap.number_streams; s++) {
+        if (ap.stream_weight_optimize[s]) ap.stream_weights[s] = v[k++];
+        for (int j = 0; j < ap.number_stream_parameters; j++)
+            if (ap.stream_optimize[s][j]) ap.stream_parameters[s][j] = v[k++];
+    }
+}
+
+struct RegionDev {
+    RegionGeom geom;
+    int n_conv = 0;
+    long n_cols = 0;
+    int blocks = 0;
+    DevBuf<double> rq, col;
+    DevBuf<RStep> rstep;
+};
+
+uint64_t hash_stars(const STAR_POINTS* sp)
+{
+    uint64_t h = 0x9e3779b97f4a7c15ull;
+    if (!sp) return h;
+    for (int i = 0; i < sp->number_stars; i++) {
+        for (int j = 0; j < 3; j++) {
+            uint64_t b;
+            std::memcpy(&b, &sp->stars[i][j], 8);
+            h = (h ^ b) * 0x100000001b3ull;
+            h ^= h >> 29;
+        }
+    }
+    return h;
+}
+
+}  // namespace
+
+struct mw_ctx {
+    int device = 0;
+    int flags = 0;
+    hipStream_t stream = nullptr;
+    int S = 0, n_conv = 0, wedge = 0, sgr = 0;
+    int n_params = 0;
+    ModelTemplate tmpl;
+    std::vector<std::unique_ptr<RegionDev>> regions;
+    long n_stars = 0;
+    DevBuf<double> star, ratio, kw;
+
+    // per-evaluation buffers
+    DevBuf<EvalConst> consts;
+    DevBuf<RegionLaunch> ritems;
+    DevBuf<StarLaunch> sitems;
+    DevBuf<ReduceItem> rred, sred;
+    DevBuf<double> ipart, spart, irows, srows, log10p;
+    DevBuf<int> first, count;
+    HostBuf<EvalConst> h_consts;
+    HostBuf<RegionLaunch> h_ritems;
+    HostBuf<StarLaunch> h_sitems;
+    HostBuf<ReduceItem> h_rred, h_sred;
+    HostBuf<double> h_irows, h_srows;
+    HostBuf<int> h_first, h_count;
+
+    hipEvent_t ev[6] = {};
+    bool have_events = false;
+    mw_timing timing{};
+    std::vector<double> last_regions;   // [n_vec][n_regions][kIntComps]
+    int last_n_vec = 0;
+
+    // drop-in cache key
+    std::vector<RegionGeom> key_geom;
+    const STAR_POINTS* key_sp = nullptr;
+    int key_stars = 0;
+    uint64_t key_hash = 0;
+
+    ~mw_ctx()
+    {
+        if (have_events)
+            for (auto& e : ev) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+// One region's calculation range inside an evaluation plan.
+struct PlanRegion {
+    int slot, region;
+    long begin, end;
+};
+
+struct Plan {
+    std::vector<EvalConst> K;              // one per slot
+    std::vector<PlanRegion> regions;       // slot-major, region order within a slot
+    bool integrals = false, stars = false, mixed = false, device_finalize = false;
+    long star_begin = 0;
+    double* log10p_host = nullptr;         // [n_slots][n_stars]
+};
+
+int build_context(mw_ctx* c, const std::vector<RegionGeom>& geoms, int n_conv, int wedge, int sgr,
+                  const STAR_POINTS* sp)
+{
+    if (n_conv < 1 || n_conv > MW_MAX_CONVOLVE || n_conv > tile_pairs())
+        return fail(MW_E_INVALID, "convolve must be between 1 and 1024");
+    if (geoms.empty()) return fail(MW_E_INVALID, "no integration regions");
+    MW_HIP_CHECK(hipGetDevice(&c->device));
+    MW_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto& e : c->ev) MW_HIP_CHECK(hipEventCreate(&e));
+    c->have_events = true;
+    c->n_conv = n_conv;
+    c->wedge = wedge;
+    c->sgr = sgr;
+    for (const RegionGeom& g : geoms) {
+        if (g.r_steps <= 0 || g.mu_steps <= 0 || g.nu_steps <= 0)
+            return fail(MW_E_INVALID, "region with a non-positive step count");
+        auto rd = std::make_unique<RegionDev>();
+        RegionTables t;
+        INTEGRAL_AREA a = area_of(g);
+        build_region_tables(a, n_conv, wedge, sgr, &t);
+        rd->geom = g;
+        rd->n_conv = n_conv;
+        rd->n_cols = t.n_cols;
+        rd->blocks = (int)((t.n_cols + kBlock - 1) / kBlock);
+        int rc;
+        if ((rc = rd->rq.reserve(t.rq.size()))) return rc;
+        if ((rc = rd->rstep.reserve(t.rstep.size()))) return rc;
+        if ((rc = rd->col.reserve(t.col.size()))) return rc;
+        MW_HIP_CHECK(hipMemcpy(rd->rq.p, t.rq.data(), sizeof(double) * t.rq.size(), hipMemcpyHostToDevice));
+        MW_HIP_CHECK(hipMemcpy(rd->rstep.p, t.rstep.data(), sizeof(RStep) * t.rstep.size(), hipMemcpyHostToDevice));
+        MW_HIP_CHECK(hipMemcpy(rd->col.p, t.col.data(), sizeof(double) * t.col.size(), hipMemcpyHostToDevice));
+        c->regions.push_back(std::move(rd));
+    }
+    c->n_stars = sp ? sp->number_stars : 0;
+    if (c->n_stars < 0) return fail(MW_E_INVALID, "negative star count");
+    std::vector<double> soa, ratio, kw;
+    if (c->n_stars > 0) {
+        build_star_soa(*sp, &soa);
+        int rc;
+        if ((rc = c->star.reserve(soa.size()))) return rc;
+        MW_HIP_CHECK(hipMemcpy(c->star.p, soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice));
+    }
+    star_conv_tables(n_conv, &ratio, &kw);
+    int rc;
+    if ((rc = c->ratio.reserve(n_conv)) || (rc = c->kw.reserve(n_conv))) return rc;
+    MW_HIP_CHECK(hipMemcpy(c->ratio.p, ratio.data(), sizeof(double) * n_conv, hipMemcpyHostToDevice));
+    MW_HIP_CHECK(hipMemcpy(c->kw.p, kw.data(), sizeof(double) * n_conv, hipMemcpyHostToDevice));
+    return MW_OK;
+}
+
+// Runs a plan: integral pass (optional), device-side region combination
+// (optional), star pass (optional).  Region rows land in c->h_irows
+// ([n_plan_regions][kIntComps]) and star rows in c->h_srows ([n_slots][kStarComps]).
+int run_plan(mw_ctx* c, Plan& P)
+{
+    const int NI = integral_comps(), NS = star_comps();
+    const int n_slots = (int)P.K.size();
+    const int n_pr = (int)P.regions.size();
+    hipStream_t st = c->stream;
+    const bool timed = (c->flags & MW_CTX_TIMING) != 0;
+    int rc;
+    MW_HIP_CHECK(hipSetDevice(c->device));
+    if ((rc = c->consts.reserve(n_slots)) || (rc = c->h_consts.reserve(n_slots))) return rc;
+    std::memcpy(c->h_consts.p, P.K.data(), sizeof(EvalConst) * n_slots);
+
+    // ---- integral work items, grouped by kernel variant ----------------------
+    struct Group { int bg; int first_item, n_items, first_block, n_blocks; };
+    std::vector<Group> igroups;
+    int total_iblocks = 0;
+    if (P.integrals && n_pr > 0) {
+        if ((rc = c->ritems.reserve(n_pr)) || (rc = c->h_ritems.reserve(n_pr)) ||
+            (rc = c->rred.reserve(n_pr)) || (rc = c->h_rred.reserve(n_pr)) ||
+            (rc = c->irows.reserve((size_t)n_pr * NI)) || (rc = c->h_irows.reserve((size_t)n_pr * NI)))
+            return rc;
+        std::vector<int> order(n_pr);
+        for (int i = 0; i < n_pr; i++) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            return P.K[P.regions[a].slot].bg_mode < P.K[P.regions[b].slot].bg_mode;
+        });
+        int item = 0;
+        for (int oi = 0; oi < n_pr; oi++) {
+            const PlanRegion& pr = P.regions[order[oi]];
+            const int bg = P.K[pr.slot].bg_mode;
+            if (igroups.empty() || igroups.back().bg != bg)
+                igroups.push_back(Group{ bg, item, 0, total_iblocks, 0 });
+            Group& g = igroups.back();
+            const RegionDev& rd = *c->regions[pr.region];
+            RegionLaunch& L = c->h_ritems.p[item];
+            L.rq = rd.rq.p;
+            L.rstep = rd.rstep.p;
+            L.col = rd.col.p;
+            L.n_cols = rd.n_cols;
+            L.r_steps = rd.geom.r_steps;
+            L.n_conv = rd.n_conv;
+            L.calc_begin = pr.begin;
+            L.calc_end = pr.end;
+            L.slot = pr.slot;
+            L.out_index = order[oi];
+            L.block_begin = g.n_blocks;          // block index within the group's launch
+            L.block_count = rd.blocks;
+            L.pad = 0;
+            ReduceItem& R = c->h_rred.p[item];
+            R.block_begin = total_iblocks;       // global partial index
+            R.block_count = rd.blocks;
+            R.out_index = order[oi];
+            R.pad = 0;
+            g.n_items++;
+            g.n_blocks += rd.blocks;
+            total_iblocks += rd.blocks;
+            item++;
+        }
+        if ((rc = c->ipart.reserve((size_t)total_iblocks * NI * 2))) return rc;
+    }
+
+    // ---- star work items -------------------------------------------------
+    std::vector<Group> sgroups;
+    int total_sblocks = 0;
+    const long n_star_pass = c->n_stars - P.star_begin;
+    if (P.stars) {
+        if ((rc = c->sitems.reserve(n_slots)) || (rc = c->h_sitems.reserve(n_slots)) ||
+            (rc = c->sred.reserve(n_slots)) || (rc = c->h_sred.reserve(n_slots)) ||
+            (rc = c->srows.reserve((size_t)n_slots * NS)) ||
+            (rc = c->h_srows.reserve((size_t)n_slots * NS)))
+            return rc;
+        const int blocks = n_star_pass > 0 ? (int)((n_star_pass + kBlock - 1) / kBlock) : 0;
+        std::vector<int> order(n_slots);
+        for (int i = 0; i < n_slots; i++) order[i] = i;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return P.K[a].bg_mode < P.K[b].bg_mode; });
+        for (int oi = 0; oi < n_slots; oi++) {
+            const int s = order[oi];
+            const int bg = P.K[s].bg_mode;
+            if (sgroups.empty() || sgroups.back().bg != bg)
+                sgroups.push_back(Group{ bg, oi, 0, total_sblocks, 0 });
+            Group& g = sgroups.back();
+            StarLaunch& L = c->h_sitems.p[oi];
+            L.star = c->star.p;
+            L.n_stars = c->n_stars;
+            L.star_begin = P.star_begin;
+            L.slot = s;
+            L.n_conv = c->n_conv;
+            L.block_begin = g.n_blocks;
+            L.block_count = blocks;
+            L.out_index = s;
+            L.pad = 0;
+            ReduceItem& R = c->h_sred.p[oi];
+            R.block_begin = total_sblocks;
+            R.block_count = blocks;
+            R.out_index = s;
+            R.pad = 0;
+            g.n_items++;
+            g.n_blocks += blocks;
+            total_sblocks += blocks;
+        }
+        if ((rc = c->spart.reserve((size_t)std::max(total_sblocks, 1) * NS * 2))) return rc;
+        if (P.log10p_host && (rc = c->log10p.reserve((size_t)n_slots * std::max<long>(c->n_stars, 1))))
+            return rc;
+    }
+
+    // ---- device-side combination of regions per slot -------------------------
+    if (P.device_finalize && P.stars && P.integrals) {
+        if ((rc = c->first.reserve(n_slots)) || (rc = c->count.reserve(n_slots)) ||
+            (rc = c->h_first.reserve(n_slots)) || (rc = c->h_count.reserve(n_slots)))
+            return rc;
+        for (int s = 0; s < n_slots; s++) {
+            c->h_first.p[s] = -1;
+            c->h_count.p[s] = 0;
+        }
+        for (int i = 0; i < n_pr; i++) {
+            const int s = P.regions[i].slot;
+            if (c->h_first.p[s] < 0) c->h_first.p[s] = i;
+            c->h_count.p[s]++;
+        }
+        for (int s = 0; s < n_slots; s++)
+            if (c->h_first.p[s] < 0) return fail(MW_E_INVALID, "slot without regions");
+        MW_HIP_CHECK(hipMemcpyAsync(c->first.p, c->h_first.p, sizeof(int) * n_slots, hipMemcpyHostToDevice, st));
+        MW_HIP_CHECK(hipMemcpyAsync(c->count.p, c->h_count.p, sizeof(int) * n_slots, hipMemcpyHostToDevice, st));
+    }
+
+    MW_HIP_CHECK(hipMemcpyAsync(c->consts.p, c->h_consts.p, sizeof(EvalConst) * n_slots, hipMemcpyHostToDevice, st));
+    if (P.integrals && n_pr > 0) {
+        MW_HIP_CHECK(hipMemcpyAsync(c->ritems.p, c->h_ritems.p, sizeof(RegionLaunch) * n_pr, hipMemcpyHostToDevice, st));
+        MW_HIP_CHECK(hipMemcpyAsync(c->rred.p, c->h_rred.p, sizeof(ReduceItem) * n_pr, hipMemcpyHostToDevice, st));
+    }
+    if (P.stars) {
+        MW_HIP_CHECK(hipMemcpyAsync(c->sitems.p, c->h_sitems.p, sizeof(StarLaunch) * n_slots, hipMemcpyHostToDevice, st));
+        MW_HIP_CHECK(hipMemcpyAsync(c->sred.p, c->h_sred.p, sizeof(ReduceItem) * n_slots, hipMemcpyHostToDevice, st));
+    }
+
+    if (timed) MW_HIP_CHECK(hipEventRecord(c->ev[0], st));
+    for (const Group& g : igroups) {
+        MW_HIP_CHECK(launch_integral(c->S, g.bg, P.mixed, c->ritems.p + g.first_item, g.n_items,
+                                     g.n_blocks, c->consts.p, c->ipart.p + (size_t)g.first_block * NI * 2, st));
+    }
+    if (timed) MW_HIP_CHECK(hipEventRecord(c->ev[1], st));
+    if (P.integrals && n_pr > 0) {
+        MW_HIP_CHECK(launch_reduce(false, c->rred.p, n_pr, c->ipart.p, c->irows.p, st));
+        if (P.device_finalize && P.stars)
+            MW_HIP_CHECK(launch_finalize(c->first.p, c->count.p, c->irows.p, c->consts.p, n_slots, st));
+    }
+    if (timed) MW_HIP_CHECK(hipEventRecord(c->ev[2], st));
+    for (const Group& g : sgroups) {
+        if (g.n_blocks == 0) continue;
+        MW_HIP_CHECK(launch_star(c->S, g.bg, P.mixed, c->sitems.p + g.first_item, g.n_items, g.n_blocks,
+                                 c->consts.p, c->ratio.p, c->kw.p,
+                                 c->spart.p + (size_t)g.first_block * NS * 2,
+                                 P.log10p_host ? c->log10p.p : nullptr, c->n_stars, st));
+    }
+    if (timed) MW_HIP_CHECK(hipEventRecord(c->ev[3], st));
+    if (P.stars) {
+        if (total_sblocks > 0)
+            MW_HIP_CHECK(launch_reduce(true, c->sred.p, n_slots, c->spart.p, c->srows.p, st));
+    }
+    if (timed) MW_HIP_CHECK(hipEventRecord(c->ev[4], st));
+    if (P.integrals && n_pr > 0)
+        MW_HIP_CHECK(hipMemcpyAsync(c->h_irows.p, c->irows.p, sizeof(double) * n_pr * NI, hipMemcpyDeviceToHost, st));
+    if (P.stars && total_sblocks > 0)
+        MW_HIP_CHECK(hipMemcpyAsync(c->h_srows.p, c->srows.p, sizeof(double) * n_slots * NS, hipMemcpyDeviceToHost, st));
+    else if (P.stars)
+        std::memset(c->h_srows.p, 0, sizeof(double) * n_slots * NS);
+    if (P.stars && P.log10p_host && c->n_stars > 0)
+        MW_HIP_CHECK(hipMemcpyAsync(P.log10p_host, c->log10p.p, sizeof(double) * n_slots * c->n_stars,
+                                    hipMemcpyDeviceToHost, st));
+    MW_HIP_CHECK(hipStreamSynchronize(st));
+    if (timed) {
+        float a = 0, b = 0, t = 0;
+        MW_HIP_CHECK(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+        MW_HIP_CHECK(hipEventElapsedTime(&b, c->ev[2], c->ev[3]));
+        MW_HIP_CHECK(hipEventElapsedTime(&t, c->ev[0], c->ev[4]));
+        c->timing.integral_ms = a;
+        c->timing.star_ms = b;
+        c->timing.total_ms = t;
+    }
+    c->timing.integral_blocks = total_iblocks;
+    c->timing.star_blocks = total_sblocks;
+    return MW_OK;
+}
+
+int make_const(const ASTRONOMY_PARAMETERS& ap, EvalConst* k)
+{
+    std::string err;
+    if (!make_eval_const(ap, k, &err)) return fail(MW_E_INVALID, err);
+    return MW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Drop-in state: one cached context per process, rebuilt when the wedge
+// geometry or the star list changes.
+// ---------------------------------------------------------------------------
+std::mutex g_dropin_mutex;
+std::unique_ptr<mw_ctx> g_dropin;
+int g_precision = -1;
+
+int dropin_precision()
+{
+    if (g_precision < 0) {
+        const char* e = std::getenv("MW_HIP_PRECISION");
+        g_precision = (e && (std::strcmp(e, "mixed") == 0 || std::strcmp(e, "1") == 0)) ? MW_PREC_MIXED
+                                                                                          : MW_PREC_FP64;
+    }
+    return g_precision;
+}
+
+int dropin_context(const ASTRONOMY_PARAMETERS* ap, const EVALUATION_STATE* es, const STAR_POINTS* sp,
+                   mw_ctx** out)
+{
+    if (!ap || !es) return fail(MW_E_INVALID, "null argument");
+    if (es->number_integrals != ap->number_integrals || ap->number_integrals < 1)
+        return fail(MW_E_INVALID, "evaluation state does not match the parameters");
+    std::vector<RegionGeom> geoms;
+    for (int g = 0; g < es->number_integrals; g++) geoms.push_back(geom_of(*es->integral[g]));
+    int dev = 0;
+    MW_HIP_CHECK(hipGetDevice(&dev));
+    const int nst = sp ? sp->number_stars : 0;
+    const uint64_t h = hash_stars(sp);
+    mw_ctx* c = g_dropin.get();
+    const bool same = c && c->device == dev && c->key_geom == geoms && c->n_conv == ap->convolve &&
+                      c->wedge == ap->wedge && c->sgr == ap->sgr_coordinates && c->key_sp == sp &&
+                      c->key_stars == nst && c->key_hash == h;
+    if (!same) {
+        g_dropin.reset();
+        auto nc = std::make_unique<mw_ctx>();
+        int rc = build_context(nc.get(), geoms, ap->convolve, ap->wedge, ap->sgr_coordinates, sp);
+        if (rc) return rc;
+        nc->key_geom = geoms;
+        nc->key_sp = sp;
+        nc->key_stars = nst;
+        nc->key_hash = h;
+        g_dropin = std::move(nc);
+    }
+    g_dropin->S = ap->number_streams;
+    *out = g_dropin.get();
+    return MW_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* mw_last_error(void) { return g_error.c_str(); }
+
+int mw_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int mw_device_synchronize(void)
+{
+    MW_HIP_CHECK(hipDeviceSynchronize());
+    return MW_OK;
+}
+
+int mw_hip_set_precision(int precision)
+{
+    if (precision != MW_PREC_FP64 && precision != MW_PREC_MIXED)
+        return fail(MW_E_INVALID, "unknown precision");
+    std::lock_guard<std::mutex> lock(g_dropin_mutex);
+    g_precision = precision;
+    return MW_OK;
+}
+
+int mw_ctx_create(const ASTRONOMY_PARAMETERS* ap, const STAR_POINTS* sp, int flags, mw_ctx** out)
+{
+    if (!ap || !out) return fail(MW_E_INVALID, "null argument");
+    *out = nullptr;
+    if (mw_device_count() <= 0) return fail(MW_E_NODEVICE, "no GPU visible");
+    if (ap->number_streams < 0 || ap->number_streams > MW_MAX_STREAMS)
+        return fail(MW_E_INVALID, "number_streams must be between 0 and 4");
+    if (ap->number_integrals < 1 || !ap->integral) return fail(MW_E_INVALID, "no integration regions");
+    std::vector<RegionGeom> geoms;
+    for (int g = 0; g < ap->number_integrals; g++) geoms.push_back(geom_of(*ap->integral[g]));
+    auto c = std::make_unique<mw_ctx>();
+    c->flags = flags;
+    int rc = build_context(c.get(), geoms, ap->convolve, ap->wedge, ap->sgr_coordinates, sp);
+    if (rc) return rc;
+    c->tmpl.load(*ap);
+    c->S = ap->number_streams;
+    c->n_params = optimized_count(c->tmpl.ap);
+    // Validate the template parameters once.
+    EvalConst k;
+    if ((rc = make_const(c->tmpl.ap, &k))) return rc;
+    *out = c.release();
+    return MW_OK;
+}
+
+int mw_ctx_parameter_count(const mw_ctx* ctx) { return ctx ? ctx->n_params : -1; }
+
+void mw_ctx_destroy(mw_ctx* ctx) { delete ctx; }
+
+size_t mw_ctx_resident_bytes(const mw_ctx* c)
+{
+    if (!c) return 0;
+    size_t b = 0;
+    for (const auto& r : c->regions)
+        b += r->rq.cap * sizeof(double) + r->rstep.cap * sizeof(RStep) + r->col.cap * sizeof(double);
+    b += (c->star.cap + c->ratio.cap + c->kw.cap) * sizeof(double);
+    return b;
+}
+
+int mw_ctx_eval(mw_ctx* c, const double* params, int n_vec, int precision, mw_result* results,
+                double* log10p)
+{
+    if (!c || n_vec < 1 || !results) return fail(MW_E_INVALID, "bad arguments");
+    if (c->n_params > 0 && !params) return fail(MW_E_INVALID, "null parameter vector");
+    if (precision != MW_PREC_FP64 && precision != MW_PREC_MIXED) return fail(MW_E_INVALID, "unknown precision");
+    const int G = (int)c->regions.size();
+    Plan P;
+    P.K.resize(n_vec);
+    for (int v = 0; v < n_vec; v++) {
+        ModelTemplate m;
+        m.load(c->tmpl.ap);
+        if (c->n_params > 0) apply_vector(m.ap, params + (size_t)v * c->n_params);
+        int rc = make_const(m.ap, &P.K[v]);
+        if (rc) return rc;
+        for (int g = 0; g < G; g++) {
+            const RegionGeom& rg = c->regions[g]->geom;
+            P.regions.push_back(PlanRegion{ v, g, 0, (long)rg.r_steps * rg.mu_steps * rg.nu_steps });
+        }
+    }
+    P.integrals = true;
+    P.stars = c->n_stars > 0;
+    P.mixed = precision == MW_PREC_MIXED;
+    P.device_finalize = true;
+    P.star_begin = 0;
+    P.log10p_host = log10p;
+    int rc = run_plan(c, P);
+    if (rc) return rc;
+    const int NI = integral_comps(), NS = star_comps();
+    c->last_regions.assign(c->h_irows.p, c->h_irows.p + (size_t)n_vec * G * NI);
+    c->last_n_vec = n_vec;
+    const double N = (double)c->n_stars;
+    for (int v = 0; v < n_vec; v++) {
+        mw_result& r = results[v];
+        std::memset(&r, 0, sizeof(r));
+        r.number_streams = c->S;
+        const double* row = c->h_irows.p + (size_t)v * G * NI;
+        r.background_integral = row[0];
+        for (int s = 0; s < c->S; s++) r.stream_integrals[s] = row[1 + s];
+        for (int g = 1; g < G; g++) {
+            r.background_integral -= row[g * NI];
+            for (int s = 0; s < c->S; s++) r.stream_integrals[s] -= row[g * NI + 1 + s];
+        }
+        if (P.stars) {
+            const double* sr = c->h_srows.p + (size_t)v * NS;
+            r.num_zero = (int)sr[NS - 1];
+            r.prob_sum = sr[0] - 238.0 * r.num_zero;
+            r.likelihood = r.prob_sum / N;
+            r.bg_only_likelihood = sr[1] / N;
+            for (int s = 0; s < c->S; s++) r.stream_only_likelihood[s] = sr[2 + s] / N;
+        } else {
+            r.likelihood = NAN;
+        }
+    }
+    return MW_OK;
+}
+
+int mw_ctx_region_integrals(const mw_ctx* c, double* out, size_t n)
+{
+    if (!c || !out) return fail(MW_E_INVALID, "null argument");
+    const size_t need = c->last_regions.size();
+    if (n < need) return fail(MW_E_INVALID, "output too small");
+    std::memcpy(out, c->last_regions.data(), sizeof(double) * need);
+    return MW_OK;
+}
+
+int mw_ctx_last_timing(const mw_ctx* c, mw_timing* out)
+{
+    if (!c || !out) return fail(MW_E_INVALID, "null argument");
+    *out = c->timing;
+    return MW_OK;
+}
+
+int mw_hip_calculate_integrals(const ASTRONOMY_PARAMETERS* ap, EVALUATION_STATE* es, const STAR_POINTS* sp)
+{
+    std::lock_guard<std::mutex> lock(g_dropin_mutex);
+    if (mw_device_count() <= 0) return fail(MW_E_NODEVICE, "no GPU visible");
+    mw_ctx* c = nullptr;
+    int rc = dropin_context(ap, es, sp, &c);
+    if (rc) return rc;
+    Plan P;
+    P.K.resize(1);
+    if ((rc = make_const(*ap, &P.K[0]))) return rc;
+    std::vector<long> ends(es->number_integrals, -1);
+    for (int g = es->current_integral; g < es->number_integrals; g++) {
+        const INTEGRAL_AREA& a = *es->integral[g];
+        const long total = (long)a.r_steps * a.mu_steps * a.nu_steps;
+        long b = a.current_calculation, e = a.max_calculation;
+        if (b < 0) return fail(MW_E_INVALID, "negative current_calculation");
+        if (b >= total) continue;            // nothing left in this region
+        // The reference computes the point at current_calculation before it
+        // tests max_calculation (evaluation_optimized.c:498-522).
+        if (e <= b) e = b + 1;
+        if (e > total) e = total;
+        ends[g] = e;
+        P.regions.push_back(PlanRegion{ 0, g, b, e });
+    }
+    P.integrals = true;
+    P.mixed = dropin_precision() == MW_PREC_MIXED;
+    if ((rc = run_plan(c, P))) return rc;
+    const int NI = integral_comps();
+    for (size_t i = 0; i < P.regions.size(); i++) {
+        INTEGRAL_AREA& a = *es->integral[P.regions[i].region];
+        const double* row = c->h_irows.p + i * NI;
+        a.background_integral += row[0];
+        for (int s = 0; s < ap->number_streams; s++) a.stream_integrals[s] += row[1 + s];
+        a.current_calculation = ends[P.regions[i].region];
+    }
+    es->current_integral = ap->number_integrals;
+    es->background_integral = es->integral[0]->background_integral;
+    for (int s = 0; s < ap->number_streams; s++) es->stream_integrals[s] = es->integral[0]->stream_integrals[s];
+    for (int g = 1; g < ap->number_integrals; g++) {
+        es->background_integral -= es->integral[g]->background_integral;
+        for (int s = 0; s < ap->number_streams; s++) es->stream_integrals[s] -= es->integral[g]->stream_integrals[s];
+    }
+    return MW_OK;
+}
+
+int mw_hip_calculate_likelihood(const ASTRONOMY_PARAMETERS* ap, EVALUATION_STATE* es, const STAR_POINTS* sp)
+{
+    std::lock_guard<std::mutex> lock(g_dropin_mutex);
+    if (!sp) return fail(MW_E_INVALID, "null star list");
+    if (mw_device_count() <= 0) return fail(MW_E_NODEVICE, "no GPU visible");
+    mw_ctx* c = nullptr;
+    int rc = dropin_context(ap, es, sp, &c);
+    if (rc) return rc;
+    Plan P;
+    P.K.resize(1);
+    if ((rc = make_const(*ap, &P.K[0]))) return rc;
+    P.K[0].I_bg = es->background_integral;
+    for (int s = 0; s < ap->number_streams; s++) P.K[0].I_st[s] = es->stream_integrals[s];
+    P.stars = true;
+    P.mixed = dropin_precision() == MW_PREC_MIXED;
+    P.star_begin = std::max(0, es->current_star_point);
+    if (P.star_begin > sp->number_stars) P.star_begin = sp->number_stars;
+    if ((rc = run_plan(c, P))) return rc;
+    const int NS = star_comps();
+    const double* sr = c->h_srows.p;
+    const int nz = (int)sr[NS - 1];
+    es->prob_sum = sr[0] - 238.0 * nz;
+    es->num_zero += nz;
+    es->current_star_point = sp->number_stars;
+    return MW_OK;
+}
+
+double mw_hip_evaluate(double* parameters, ASTRONOMY_PARAMETERS* ap, EVALUATION_STATE* es, STAR_POINTS* sp)
+{
+    // set_astronomy_parameters + reset_evaluation_state (evaluation_state.c:178-199)
+    apply_vector(*ap, parameters);
+    es->current_integral = 0;
+    es->background_integral = 0;
+    for (int s = 0; s < es->number_streams; s++) es->stream_integrals[s] = 0;
+    es->current_star_point = 0;
+    es->num_zero = 0;
+    es->bad_jacobians = 0;
+    es->prob_sum = 0;
+    for (int g = 0; g < es->number_integrals; g++) {
+        es->integral[g]->background_integral = 0;
+        for (int s = 0; s < es->integral[g]->number_streams; s++) es->integral[g]->stream_integrals[s] = 0;
+        es->integral[g]->current_calculation = es->integral[g]->min_calculation;
+    }
+    int rc = mw_hip_calculate_integrals(ap, es, sp);
+    if (rc) {
+        fprintf(stderr, "APP: error calculating integrals: %d (%s)\n", rc, mw_last_error());
+        exit(rc);
+    }
+    rc = mw_hip_calculate_likelihood(ap, es, sp);
+    if (rc) {
+        fprintf(stderr, "APP: error calculating likelihood: %d (%s)\n", rc, mw_last_error());
+        exit(rc);
+    }
+    return es->prob_sum / (sp->number_stars - es->bad_jacobians);
+}
+
+}  // extern "C"

@@ -1,0 +1,111 @@
+// mw_internal.h -- shared declarations of the HIP separation library.
+//
+// Layouts of everything the kernels read: per-wedge resident tables
+// (parameter independent), per-evaluation constants (one slot per parameter
+// vector), and the work descriptors of a launch.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "mw_separation_types.h"
+
+namespace mw {
+
+constexpr int kMaxStreams = 4;        // streams per kernel instantiation
+constexpr int kBlock = 256;           // work-items per workgroup (4 wavefronts)
+constexpr double kSunR0 = 8.5;        // evaluation_optimized.c:43, stCoords.c:30
+constexpr double kStdev = 0.6;        // evaluation_optimized.c:41
+constexpr double kAbsM = 4.2;         // evaluation_optimized.c:44
+
+// Background profile selected per evaluation.
+enum BgMode : int {
+    kBgHernquist = 0,     // alpha == delta == 1, no aux profile
+    kBgHernquistAux = 1,  // alpha == delta == 1 with a g^2 + b g + c
+    kBgGeneral = 2,       // pow(rg, alpha) * pow(rg + r0, 3 - alpha + delta)
+    kBgZeroQ = 3,         // q == 0: the background is -1 per r-step
+};
+
+// Per-r-step table entry of one region (host-built, FP64).
+struct RStep {
+    double irv;           // (r_{k+1}^3 - r_k^3) / 3 * dmu (radians)
+    double reff_xr_rp3;   // reff(r') * 3 sigma / r'^3
+    double q0, q1, q2;    // sum_i qw_i, sum_i qw_i g_i, sum_i qw_i g_i^2 (aux profile)
+    double pad[3];
+};
+static_assert(sizeof(RStep) == 64, "RStep is one 64-byte row");
+
+// Host copy of one region's tables.
+struct RegionTables {
+    int r_steps = 0, mu_steps = 0, nu_steps = 0, n_conv = 0;
+    long n_cols = 0;
+    std::vector<double> rq;        // [r_steps][n_conv][2]: (r_point, qw_r3_N)
+    std::vector<RStep> rstep;      // [r_steps]
+    std::vector<double> col;       // [4][n_cols] SoA: cos b cos l, cos b sin l, sin b, ids
+};
+
+// Per-evaluation constants, one slot per parameter vector (device resident).
+struct alignas(64) EvalConst {
+    // background
+    double q, inv_q2, r0, alpha, alpha_delta3, bg_a, bg_b, bg_c;
+    int bg_mode, n_streams, pad0, pad1;
+    // streams: centre c (galactocentric kpc), unit axis a, 1 / (2 sigma^2), on/off
+    double sc[kMaxStreams][3];
+    double sa[kMaxStreams][3];
+    double inv_2s2[kMaxStreams];
+    double on[kMaxStreams];
+    // likelihood weights: exp(w_b), exp(w_s), 0.001 * (exp(w_b) + sum exp(w_s))
+    double exp_bg_w, exp_st_w[kMaxStreams], sum_exp_w;
+    // integrals (filled by the integral pass, read by the star pass)
+    double I_bg, I_st[kMaxStreams];
+};
+
+// One region of one evaluation inside a launch.
+struct RegionLaunch {
+    const double* rq;          // [R][n][2]
+    const RStep* rstep;        // [R]
+    const double* col;         // [4][n_cols]
+    long n_cols;
+    int r_steps, n_conv;
+    long calc_begin, calc_end; // flat index range [begin, end), mu-major, nu, r
+    int slot;                  // evaluation-constant slot
+    int out_index;             // index into the per-region result rows
+    int block_begin, block_count;
+    int pad;
+};
+
+// Stars resident on the device, SoA.
+struct StarLaunch {
+    const double* star;        // [4][n]: cos b cos l, cos b sin l, sin b, r (kpc)
+    long n_stars;
+    long star_begin;           // first star of this pass
+    int slot, n_conv;
+    int block_begin, block_count;
+    int out_index;
+    int pad;
+};
+
+// Second-pass item: the partials of blocks [block_begin, block_begin +
+// block_count) summed in block order into row out_index.
+struct ReduceItem {
+    int block_begin, block_count, out_index, pad;
+};
+
+// Host-side table builders (mw_tables.cpp); all FP64 with glibc libm.
+void gauss_legendre(double x1, double x2, double* x, double* w, int n);
+void gc_to_lb(int wedge, int sgr, double mu, double nu, double* l, double* b);
+void lbr_to_xyz(const double* lbr, double* xyz);
+void build_region_tables(const INTEGRAL_AREA& ia, int n_conv, int wedge, int sgr,
+                         RegionTables* out);
+void build_star_soa(const STAR_POINTS& sp, std::vector<double>* out);
+// Fills an EvalConst from the model parameters held in ap; returns false
+// (with a message) if the parameters cannot be evaluated here.
+bool make_eval_const(const ASTRONOMY_PARAMETERS& ap, EvalConst* k, std::string* err);
+
+// Star-pass constants shared by every star of a context: r_i / r_star and
+// W_i * coeff * exp(-dx_i^2 / (2 sigma^2)).
+void star_conv_tables(int n_conv, std::vector<double>* ratio, std::vector<double>* kw);
+
+}  // namespace mw

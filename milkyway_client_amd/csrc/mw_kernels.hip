@@ -1,0 +1,483 @@
+// mw_kernels.hip -- gfx950 kernels of the separation likelihood.
+//
+//   integral_kernel   the volume integral over the (mu, nu, r) wedge grid
+//                     (reference: evaluation_optimized.c:424-562 around the
+//                     integrand :190-299).  One work-item per (mu, nu) column
+//                     streams over every r-step of its region; the r-step
+//                     tables (r_point, qw_r3_N) are staged through LDS in
+//                     r-tiles and read as wavefront-wide broadcasts; each
+//                     column keeps Kahan sums in registers; a block reduces its
+//                     256 columns with double-double (TwoSum) arithmetic over
+//                     DPP/permute shuffles and LDS.
+//   star_kernel       the star pass (evaluation_optimized.c:613-753): one
+//                     work-item per star, SoA star records, per-star log10 P.
+//   reduce_kernel     second pass: each work item's per-block partials summed
+//                     in block order (double-double), so the result does not
+//                     depend on scheduling.
+//   finalize_kernel   I = I_main - sum I_cut per evaluation slot, written into
+//                     the slot's constants for the star pass.
+//
+// Precision modes: MIXED == false is plain FP64 throughout.  MIXED == true
+// keeps FP64 for the positions (lbr -> xyz), the squared stream-axis distance,
+// the background's squared radius, every accumulation and the per-star log,
+// and evaluates sqrt, the reciprocal and exp in FP32 on the transcendental
+// unit (v_rsq_f32, v_rcp_f32, v_exp_f32).
+#include <hip/hip_runtime.h>
+
+#include "mw_internal.h"
+
+namespace mw {
+
+constexpr int kTileD2 = 1024;           // LDS r-tile: 1024 (r_point, qw) pairs = 16 KiB
+constexpr int kIntComps = 1 + kMaxStreams;
+constexpr int kStarComps = 2 + kMaxStreams + 1;   // prob, bg-only, st-only[4], zero count
+
+// ---------------------------------------------------------------------------
+// double-double helpers (Knuth TwoSum, Dekker FastTwoSum)
+// ---------------------------------------------------------------------------
+struct DD {
+    double hi, lo;
+};
+
+__device__ __forceinline__ DD dd_add(DD a, DD b)
+{
+    const double s = a.hi + b.hi;
+    const double bb = s - a.hi;
+    double e = (a.hi - (s - bb)) + (b.hi - bb);
+    e += a.lo + b.lo;
+    const double h = s + e;
+    return DD{ h, e - (h - s) };
+}
+
+__device__ __forceinline__ DD dd_shfl_xor(DD v, int mask)
+{
+    return DD{ __shfl_xor(v.hi, mask, 64), __shfl_xor(v.lo, mask, 64) };
+}
+
+// Sum one DD value per work-item over the workgroup; result valid in thread 0.
+template <int NC>
+__device__ __forceinline__ void block_reduce_dd(DD (&v)[NC], DD (*lds)[NC])
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) v[c] = dd_add(v[c], dd_shfl_xor(v[c], m));
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) lds[wave][c] = v[c];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            DD a = lds[0][c];
+            for (int w = 1; w < kBlock / 64; w++) a = dd_add(a, lds[w][c]);
+            v[c] = a;
+        }
+    }
+}
+
+__device__ __forceinline__ void kahan_add(double& s, double& c, double x)
+{
+    const double t = s;
+    s += x;
+    c += x - (s - t);
+}
+
+// Work item owning block b: the last item whose block_begin <= b.
+template <typename T>
+__device__ __forceinline__ int find_item(const T* __restrict__ items, int n_items, int b)
+{
+    int lo = 0, hi = n_items - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].block_begin <= b) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// ---------------------------------------------------------------------------
+// The integrand at one convolution point (position r along the column's unit
+// line of sight u), accumulated into bg and st[].  Restates
+// evaluation_optimized.c:222-296 per point.
+// ---------------------------------------------------------------------------
+template <int S, int BG, bool MIXED>
+__device__ __forceinline__ void point_terms(double r, double qw, double ux, double uy, double uz,
+                                            const EvalConst& K, double& bg, double (&st)[kMaxStreams])
+{
+    const double x = r * ux - kSunR0;
+    const double y = r * uy;
+    const double z = r * uz;
+    if constexpr (BG != kBgZeroQ) {
+        const double rg2 = x * x + y * y + z * z * K.inv_q2;
+        if constexpr (MIXED) {
+            const float g2 = (float)rg2;
+            const float ri = __builtin_amdgcn_rsqf(g2);
+            const float rg = g2 * ri;
+            const float rs = rg + (float)K.r0;
+            const float den = rg * rs * rs * rs;
+            bg = fma(qw, (double)__builtin_amdgcn_rcpf(den), bg);
+        } else {
+            const double rg = sqrt(rg2);
+            if constexpr (BG == kBgGeneral) {
+                bg += qw / (pow(rg, K.alpha) * pow(rg + K.r0, K.alpha_delta3));
+            } else {
+                const double rs = rg + K.r0;
+                bg += qw / (rg * rs * rs * rs);
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        double v0 = x - K.sc[s][0];
+        double v1 = y - K.sc[s][1];
+        double v2 = z - K.sc[s][2];
+        const double d = K.sa[s][0] * v0 + K.sa[s][1] * v1 + K.sa[s][2] * v2;
+        v0 -= d * K.sa[s][0];
+        v1 -= d * K.sa[s][1];
+        v2 -= d * K.sa[s][2];
+        const double n2 = v0 * v0 + v1 * v1 + v2 * v2;
+        if constexpr (MIXED) {
+            const float e = __expf((float)(-n2 * K.inv_2s2[s]));
+            st[s] = fma(qw, (double)e, st[s]);
+        } else {
+            st[s] += qw * exp(-n2 * K.inv_2s2[s]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Integral kernel
+// ---------------------------------------------------------------------------
+template <int S, int BG, bool MIXED>
+__global__ __launch_bounds__(kBlock) void integral_kernel(const RegionLaunch* __restrict__ items,
+                                                          int n_items,
+                                                          const EvalConst* __restrict__ consts,
+                                                          double* __restrict__ partials)
+{
+    __shared__ double2 tile[kTileD2];
+    __shared__ DD red[kBlock / 64][kIntComps];
+
+    const int b = blockIdx.x;
+    const RegionLaunch& it = items[find_item(items, n_items, b)];
+    const EvalConst& K = consts[it.slot];
+    const int R = it.r_steps, n = it.n_conv;
+    const long C = it.n_cols;
+    const long col0 = (long)(b - it.block_begin) * kBlock;
+    const long col = col0 + threadIdx.x;
+    const bool valid = col < C;
+    const long cc = valid ? col : C - 1;
+
+    DD out[kIntComps];
+#pragma unroll
+    for (int c = 0; c < kIntComps; c++) out[c] = DD{ 0.0, 0.0 };
+
+    // Blocks whose columns lie wholly outside [calc_begin, calc_end) add nothing.
+    const long last = (col0 + kBlock < C ? col0 + kBlock : C);
+    const bool active = it.calc_end > col0 * R && it.calc_begin < last * R;
+    if (active) {
+        const double ux = it.col[cc];
+        const double uy = it.col[C + cc];
+        const double uz = it.col[2 * C + cc];
+        const double wnu = it.col[3 * C + cc];
+        const long base = cc * (long)R;
+        long lo_l = it.calc_begin - base, hi_l = it.calc_end - base;
+        const int r_lo = valid ? (int)(lo_l < 0 ? 0 : (lo_l > R ? R : lo_l)) : 0;
+        const int r_hi = valid ? (int)(hi_l < 0 ? 0 : (hi_l > R ? R : hi_l)) : 0;
+
+        double acc[1 + S], cmp[1 + S];
+#pragma unroll
+        for (int c = 0; c <= S; c++) acc[c] = cmp[c] = 0.0;
+
+        const int rt = kTileD2 / n;   // r-steps per LDS tile (host guarantees n <= kTileD2)
+        const double2* rq = reinterpret_cast<const double2*>(it.rq);
+        for (int k0 = 0; k0 < R; k0 += rt) {
+            const int kt = (R - k0 < rt) ? R - k0 : rt;
+            __syncthreads();
+            for (int e = threadIdx.x; e < kt * n; e += kBlock) tile[e] = rq[(long)k0 * n + e];
+            __syncthreads();
+            for (int kk = 0; kk < kt; kk++) {
+                const int k = k0 + kk;
+                double bg = 0.0;
+                double st[kMaxStreams] = { 0.0, 0.0, 0.0, 0.0 };
+                const double2* row = tile + kk * n;
+#pragma unroll 2
+                for (int i = 0; i < n; i++) {
+                    const double2 p = row[i];
+                    point_terms<S, BG, MIXED>(p.x, p.y, ux, uy, uz, K, bg, st);
+                }
+                const RStep& rs = it.rstep[k];
+                double f = rs.reff_xr_rp3 * rs.irv;
+                double bgv;
+                if constexpr (BG == kBgZeroQ) {
+                    bgv = -rs.irv;
+                } else {
+                    if constexpr (BG == kBgHernquistAux)
+                        bg += K.bg_a * rs.q2 + K.bg_b * rs.q1 + K.bg_c * rs.q0;
+                    bgv = bg * f;
+                }
+                if (k >= r_lo && k < r_hi) {
+                    kahan_add(acc[0], cmp[0], bgv);
+#pragma unroll
+                    for (int s = 0; s < S; s++) kahan_add(acc[1 + s], cmp[1 + s], st[s] * f);
+                }
+            }
+        }
+        if (valid) {
+            out[0] = DD{ (acc[0] + cmp[0]) * wnu, 0.0 };
+#pragma unroll
+            for (int s = 0; s < S; s++) out[1 + s] = DD{ (acc[1 + s] + cmp[1 + s]) * wnu * K.on[s], 0.0 };
+        }
+    }
+    block_reduce_dd<kIntComps>(out, red);
+    if (threadIdx.x == 0) {
+        double* p = partials + (long)b * kIntComps * 2;
+#pragma unroll
+        for (int c = 0; c < kIntComps; c++) {
+            p[2 * c] = out[c].hi;
+            p[2 * c + 1] = out[c].lo;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Star kernel
+// ---------------------------------------------------------------------------
+template <int S, int BG, bool MIXED>
+__global__ __launch_bounds__(kBlock) void star_kernel(const StarLaunch* __restrict__ items, int n_items,
+                                                      const EvalConst* __restrict__ consts,
+                                                      const double* __restrict__ conv_ratio,
+                                                      const double* __restrict__ conv_kw,
+                                                      double* __restrict__ partials,
+                                                      double* __restrict__ log10p, long log10p_stride)
+{
+    __shared__ DD red[kBlock / 64][kStarComps];
+    const int b = blockIdx.x;
+    const StarLaunch& it = items[find_item(items, n_items, b)];
+    const EvalConst& K = consts[it.slot];
+    const long N = it.n_stars;
+    const long idx = it.star_begin + (long)(b - it.block_begin) * kBlock + threadIdx.x;
+    const bool valid = idx < N;
+    const long i = valid ? idx : N - 1;
+    const int n = it.n_conv;
+
+    const double ux = it.star[i];
+    const double uy = it.star[N + i];
+    const double uz = it.star[2 * N + i];
+    const double rstar = it.star[3 * N + i];
+    const double reff = it.star[4 * N + i];
+
+    double bg = 0.0;
+    double st[kMaxStreams] = { 0.0, 0.0, 0.0, 0.0 };
+    for (int j = 0; j < n; j++) {
+        const double r = rstar * conv_ratio[j];
+        const double qw = conv_kw[j] * (r * r * r);
+        point_terms<S, BG, MIXED>(r, qw, ux, uy, uz, K, bg, st);
+    }
+    if constexpr (BG == kBgZeroQ) {
+        bg = -1.0;
+    } else {
+        if constexpr (BG == kBgHernquistAux) {
+            for (int j = 0; j < n; j++) {
+                // aux profile term qw (a g^2 + b g + c) with g = g' + dx_j
+                const double r = rstar * conv_ratio[j];
+                const double qw = conv_kw[j] * (r * r * r);
+                const double g = 5.0 * (log10(r * 1000.0) - 1.0) + kAbsM;
+                bg += qw * (K.bg_a * (g * g) + K.bg_b * g + K.bg_c);
+            }
+        }
+        bg *= reff;
+    }
+
+    // Mixture (evaluation_optimized.c:674-716).
+    DD out[kStarComps];
+#pragma unroll
+    for (int c = 0; c < kStarComps; c++) out[c] = DD{ 0.0, 0.0 };
+    double bg_only = (bg / K.I_bg) * K.exp_bg_w;
+    double p = bg_only;
+    out[1].hi = (bg_only == 0.0) ? -238.0 : log10(bg_only / K.sum_exp_w);
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const double so = (st[s] * reff * K.on[s]) / K.I_st[s] * K.exp_st_w[s];
+        p += so;
+        out[2 + s].hi = (so == 0.0) ? -238.0 : log10(so / K.sum_exp_w);
+    }
+    p /= K.sum_exp_w;
+    double lp;
+    if (p != 0.0) {
+        lp = log10(p);
+        out[0].hi = lp;
+    } else {
+        lp = -238.0;
+        out[kStarComps - 1].hi = 1.0;
+    }
+    if (!valid) {
+#pragma unroll
+        for (int c = 0; c < kStarComps; c++) out[c] = DD{ 0.0, 0.0 };
+    } else if (log10p) {
+        log10p[(long)it.out_index * log10p_stride + idx] = lp;
+    }
+    block_reduce_dd<kStarComps>(out, red);
+    if (threadIdx.x == 0) {
+        double* q = partials + (long)b * kStarComps * 2;
+#pragma unroll
+        for (int c = 0; c < kStarComps; c++) {
+            q[2 * c] = out[c].hi;
+            q[2 * c + 1] = out[c].lo;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Second pass: per work item, the sum of its blocks' partials in block order.
+// rows[out_index][c] = hi + lo.
+// ---------------------------------------------------------------------------
+template <int NC>
+__global__ __launch_bounds__(kBlock) void reduce_kernel(const ReduceItem* __restrict__ items,
+                                                        const double* __restrict__ partials,
+                                                        double* __restrict__ rows)
+{
+    __shared__ DD red[kBlock / 64][NC];
+    const ReduceItem it = items[blockIdx.x];
+    DD v[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) v[c] = DD{ 0.0, 0.0 };
+    for (int j = threadIdx.x; j < it.block_count; j += kBlock) {
+        const double* p = partials + (long)(it.block_begin + j) * NC * 2;
+#pragma unroll
+        for (int c = 0; c < NC; c++) v[c] = dd_add(v[c], DD{ p[2 * c], p[2 * c + 1] });
+    }
+    block_reduce_dd<NC>(v, red);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) rows[(long)it.out_index * NC + c] = v[c].hi + v[c].lo;
+    }
+}
+
+// I = I_region0 - I_region1 - ... per slot (evaluation_optimized.c:584-593),
+// written into the slot's constants.  slot_rows[s] = first row of slot s;
+// rows of one slot are consecutive, region 0 first.
+__global__ void finalize_kernel(const int* __restrict__ slot_first_row,
+                                const int* __restrict__ slot_n_rows,
+                                const double* __restrict__ rows, EvalConst* consts, int n_slots)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_slots) return;
+    const double* r = rows + (long)slot_first_row[s] * kIntComps;
+    double bg = r[0];
+    double st[kMaxStreams];
+    for (int j = 0; j < kMaxStreams; j++) st[j] = r[1 + j];
+    for (int g = 1; g < slot_n_rows[s]; g++) {
+        bg -= r[g * kIntComps];
+        for (int j = 0; j < kMaxStreams; j++) st[j] -= r[g * kIntComps + 1 + j];
+    }
+    consts[s].I_bg = bg;
+    for (int j = 0; j < kMaxStreams; j++) consts[s].I_st[j] = st[j];
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+using IntegralFn = void (*)(const RegionLaunch*, int, const EvalConst*, double*);
+using StarFn = void (*)(const StarLaunch*, int, const EvalConst*, const double*, const double*,
+                        double*, double*, long);
+
+template <bool MIXED, int BG>
+IntegralFn integral_fn(int S)
+{
+    switch (S) {
+    case 0: return integral_kernel<0, BG, MIXED>;
+    case 1: return integral_kernel<1, BG, MIXED>;
+    case 2: return integral_kernel<2, BG, MIXED>;
+    case 3: return integral_kernel<3, BG, MIXED>;
+    default: return integral_kernel<4, BG, MIXED>;
+    }
+}
+
+template <bool MIXED, int BG>
+StarFn star_fn(int S)
+{
+    switch (S) {
+    case 0: return star_kernel<0, BG, MIXED>;
+    case 1: return star_kernel<1, BG, MIXED>;
+    case 2: return star_kernel<2, BG, MIXED>;
+    case 3: return star_kernel<3, BG, MIXED>;
+    default: return star_kernel<4, BG, MIXED>;
+    }
+}
+
+// Mixed precision is instantiated for the Hernquist profile; the other
+// background modes run the FP64 kernels.
+static IntegralFn pick_integral(int S, int bg, bool mixed)
+{
+    if (mixed && bg == kBgHernquist) return integral_fn<true, kBgHernquist>(S);
+    switch (bg) {
+    case kBgHernquist: return integral_fn<false, kBgHernquist>(S);
+    case kBgHernquistAux: return integral_fn<false, kBgHernquistAux>(S);
+    case kBgGeneral: return integral_fn<false, kBgGeneral>(S);
+    default: return integral_fn<false, kBgZeroQ>(S);
+    }
+}
+
+static StarFn pick_star(int S, int bg, bool mixed)
+{
+    if (mixed && bg == kBgHernquist) return star_fn<true, kBgHernquist>(S);
+    switch (bg) {
+    case kBgHernquist: return star_fn<false, kBgHernquist>(S);
+    case kBgHernquistAux: return star_fn<false, kBgHernquistAux>(S);
+    case kBgGeneral: return star_fn<false, kBgGeneral>(S);
+    default: return star_fn<false, kBgZeroQ>(S);
+    }
+}
+
+hipError_t launch_integral(int S, int bg, bool mixed, const RegionLaunch* d_items, int n_items,
+                           int n_blocks, const EvalConst* d_consts, double* d_partials,
+                           hipStream_t stream)
+{
+    if (n_blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(pick_integral(S, bg, mixed), dim3(n_blocks), dim3(kBlock), 0, stream,
+                       d_items, n_items, d_consts, d_partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_star(int S, int bg, bool mixed, const StarLaunch* d_items, int n_items,
+                       int n_blocks, const EvalConst* d_consts, const double* d_ratio,
+                       const double* d_kw, double* d_partials, double* d_log10p, long stride,
+                       hipStream_t stream)
+{
+    if (n_blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(pick_star(S, bg, mixed), dim3(n_blocks), dim3(kBlock), 0, stream, d_items,
+                       n_items, d_consts, d_ratio, d_kw, d_partials, d_log10p, stride);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce(bool star, const void* d_items, int n_items, const double* d_partials,
+                         double* d_rows, hipStream_t stream)
+{
+    if (n_items <= 0) return hipSuccess;
+    const ReduceItem* it = static_cast<const ReduceItem*>(d_items);
+    if (star)
+        hipLaunchKernelGGL(reduce_kernel<kStarComps>, dim3(n_items), dim3(kBlock), 0, stream, it,
+                           d_partials, d_rows);
+    else
+        hipLaunchKernelGGL(reduce_kernel<kIntComps>, dim3(n_items), dim3(kBlock), 0, stream, it,
+                           d_partials, d_rows);
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize(const int* d_first, const int* d_count, const double* d_rows,
+                           EvalConst* d_consts, int n_slots, hipStream_t stream)
+{
+    if (n_slots <= 0) return hipSuccess;
+    hipLaunchKernelGGL(finalize_kernel, dim3((n_slots + 63) / 64), dim3(64), 0, stream, d_first,
+                       d_count, d_rows, d_consts, n_slots);
+    return hipGetLastError();
+}
+
+int tile_pairs() { return kTileD2; }
+int integral_comps() { return kIntComps; }
+int star_comps() { return kStarComps; }
+
+}  // namespace mw
