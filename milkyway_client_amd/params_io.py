@@ -70,6 +70,15 @@ def _triple(lines: _Lines, key: str) -> Tuple[float, float, int]:
 
 def parse_astronomy_parameters(text: str) -> AstronomyParameters:
     """Parse an astronomy parameter file (parameters.c:107-228)."""
+    try:
+        return _parse(text)
+    except ParameterFileError:
+        raise
+    except (ValueError, IndexError) as e:
+        raise ParameterFileError(str(e)) from e
+
+
+def _parse(text: str) -> AstronomyParameters:
     L = _Lines(text)
     version = 0.01
     if L.peek().startswith("parameters_version"):

@@ -217,3 +217,11 @@ def evaluate(params: ParamsHolder, stars: StarsHolder, vector) -> float:
 
 def omp_threads() -> int:
     return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+def wedge_incl(wedge: int) -> float:
+    return lib().orc_wedge_incl(wedge)
+
+
+def get_node() -> float:
+    return lib().orc_get_node()
