@@ -56,6 +56,12 @@ struct alignas(64) EvalConst {
     double sa[kMaxStreams][3];
     double inv_2s2[kMaxStreams];
     double on[kMaxStreams];
+    // Stream-axis distance in linear form.  With c' = c + (8.5, 0, 0) the
+    // offset of a point r u - (8.5, 0, 0) from the axis is r u_perp - c_perp,
+    // where x_perp = x - (x . a) a.  Both are scaled by k = 1 / (sigma sqrt 2),
+    // so |r u_perp k - c_perp k|^2 is the exponent |v_perp|^2 / (2 sigma^2).
+    double ks[kMaxStreams];
+    double cperp[kMaxStreams][3];
     // likelihood weights: exp(w_b), exp(w_s), 0.001 * (exp(w_b) + sum exp(w_s))
     double exp_bg_w, exp_st_w[kMaxStreams], sum_exp_w;
     // integrals (filled by the integral pass, read by the star pass)
@@ -78,7 +84,8 @@ struct RegionLaunch {
 
 // Stars resident on the device, SoA.
 struct StarLaunch {
-    const double* star;        // [4][n]: cos b cos l, cos b sin l, sin b, r (kpc)
+    const double* star;        // [5][n]: cos b cos l, cos b sin l, sin b, r (kpc),
+                               //         reff(r) * 3 sigma / r^3
     long n_stars;
     long star_begin;           // first star of this pass
     int slot, n_conv;

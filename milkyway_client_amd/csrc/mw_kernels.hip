@@ -17,11 +17,12 @@
 //   finalize_kernel   I = I_main - sum I_cut per evaluation slot, written into
 //                     the slot's constants for the star pass.
 //
-// Precision modes: MIXED == false is plain FP64 throughout.  MIXED == true
-// keeps FP64 for the positions (lbr -> xyz), the squared stream-axis distance,
-// the background's squared radius, every accumulation and the per-star log,
-// and evaluates sqrt, the reciprocal and exp in FP32 on the transcendental
-// unit (v_rsq_f32, v_rcp_f32, v_exp_f32).
+// Precision modes: MIXED == false is FP64 throughout (exp, 1/sqrt and 1/x
+// from hardware estimates refined to full FP64 accuracy).  MIXED == true
+// keeps FP64 for the positions (lbr -> xyz), the squared stream-axis
+// distance, the background's squared radius, every accumulation and the
+// per-star log, and evaluates sqrt, the reciprocal and exp in FP32 on the
+// transcendental unit (v_rsq_f32, v_rcp_f32, v_exp_f32).
 #include <hip/hip_runtime.h>
 
 #include "mw_internal.h"
@@ -99,51 +100,141 @@ __device__ __forceinline__ int find_item(const T* __restrict__ items, int n_item
 }
 
 // ---------------------------------------------------------------------------
-// The integrand at one convolution point (position r along the column's unit
-// line of sight u), accumulated into bg and st[].  Restates
-// evaluation_optimized.c:222-296 per point.
+// FP64 elementary functions used by the integrand.
+// ---------------------------------------------------------------------------
+
+// e^x for x <= 0, the exponent of a Gaussian tube density.  Cody-Waite
+// reduction x = k ln 2 + r with |r| <= ln(2) / 2 (ln 2 split into a double and
+// its remainder), then the degree-12 Taylor polynomial of e^r (truncation
+// error below 2e-16 relative) and 2^k by ldexp: about 3 ulp overall.
+// Arguments below -1000 are clamped; e^-1000 underflows to 0 exactly as the
+// libm exp does for every argument below -745.2.
+__device__ __forceinline__ double exp_neg(double x)
+{
+    x = fmax(x, -1000.0);
+    const double k = rint(x * 1.4426950408889634074);
+    double r = fma(-k, 6.93147180559945286227e-01, x);
+    r = fma(-k, 2.31904681384629955842e-17, r);
+    double p = 1.0 / 479001600.0;
+    p = fma(p, r, 1.0 / 39916800.0);
+    p = fma(p, r, 1.0 / 3628800.0);
+    p = fma(p, r, 1.0 / 362880.0);
+    p = fma(p, r, 1.0 / 40320.0);
+    p = fma(p, r, 1.0 / 5040.0);
+    p = fma(p, r, 1.0 / 720.0);
+    p = fma(p, r, 1.0 / 120.0);
+    p = fma(p, r, 1.0 / 24.0);
+    p = fma(p, r, 1.0 / 6.0);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return ldexp(p, (int)k);
+}
+
+// 1/sqrt(a), a > 0: the hardware estimate refined by two Newton steps
+// y <- y + y (1 - a y^2) / 2 (error squares each step; ~1 ulp after two).
+__device__ __forceinline__ double rsqrt_nr(double a)
+{
+    double y = __builtin_amdgcn_rsq(a);
+#pragma unroll
+    for (int it = 0; it < 2; it++) {
+        const double e = fma(-(a * y), y, 1.0);
+        y = fma(0.5 * y, e, y);
+    }
+    return y;
+}
+
+// 1/d, d > 0: the hardware estimate refined by two Newton steps
+// y <- y + y (1 - d y).
+__device__ __forceinline__ double rcp_nr(double d)
+{
+    double y = __builtin_amdgcn_rcp(d);
+#pragma unroll
+    for (int it = 0; it < 2; it++) {
+        const double e = fma(-d, y, 1.0);
+        y = fma(y, e, y);
+    }
+    return y;
+}
+
+// ---------------------------------------------------------------------------
+// One line of sight: its unit vector u and, per stream, the part of u
+// perpendicular to the stream axis scaled by 1 / (sigma sqrt 2).  A point at
+// distance r along the sight sits at r u - (8.5, 0, 0); its offset from the
+// stream axis is r u_perp - c_perp (evaluation_optimized.c:283-293 computes
+// the same vector as v - (v . a) a), so the Gaussian exponent is
+// |r u_perp k - c_perp k|^2: three FMAs and a dot product per point.
+// ---------------------------------------------------------------------------
+template <int S>
+struct Sight {
+    double ux, uy, uz;
+    double up[S > 0 ? S : 1][3];
+};
+
+template <int S>
+__device__ __forceinline__ Sight<S> make_sight(double ux, double uy, double uz, const EvalConst& K)
+{
+    Sight<S> v;
+    v.ux = ux;
+    v.uy = uy;
+    v.uz = uz;
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const double d = ux * K.sa[s][0] + uy * K.sa[s][1] + uz * K.sa[s][2];
+        v.up[s][0] = (ux - d * K.sa[s][0]) * K.ks[s];
+        v.up[s][1] = (uy - d * K.sa[s][1]) * K.ks[s];
+        v.up[s][2] = (uz - d * K.sa[s][2]) * K.ks[s];
+    }
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// The integrand at one convolution point (distance r along the sight, weight
+// qw = W_i r^3 N(dx_i)), accumulated into bg and st[]
+// (evaluation_optimized.c:222-296 per point).
 // ---------------------------------------------------------------------------
 template <int S, int BG, bool MIXED>
-__device__ __forceinline__ void point_terms(double r, double qw, double ux, double uy, double uz,
+__device__ __forceinline__ void point_terms(double r, double qw, const Sight<S>& v,
                                             const EvalConst& K, double& bg, double (&st)[kMaxStreams])
 {
-    const double x = r * ux - kSunR0;
-    const double y = r * uy;
-    const double z = r * uz;
     if constexpr (BG != kBgZeroQ) {
-        const double rg2 = x * x + y * y + z * z * K.inv_q2;
+        const double x = fma(r, v.ux, -kSunR0);
+        const double y = r * v.uy;
+        const double z = r * v.uz;
+        const double rg2 = fma(x, x, fma(y, y, z * z * K.inv_q2));
         if constexpr (MIXED) {
-            const float g2 = (float)rg2;
-            const float ri = __builtin_amdgcn_rsqf(g2);
-            const float rg = g2 * ri;
-            const float rs = rg + (float)K.r0;
-            const float den = rg * rs * rs * rs;
-            bg = fma(qw, (double)__builtin_amdgcn_rcpf(den), bg);
-        } else {
+            // FP32 estimates of 1/sqrt and 1/x, one FP64 Newton step each
+            double rinv = (double)__builtin_amdgcn_rsqf((float)rg2);
+            rinv = fma(0.5 * rinv, fma(-(rg2 * rinv), rinv, 1.0), rinv);
+            const double rs = fma(rg2, rinv, K.r0);
+            double irs = (double)__builtin_amdgcn_rcpf((float)rs);
+            irs = fma(irs, fma(-rs, irs, 1.0), irs);
+            bg = fma(qw * rinv, irs * irs * irs, bg);
+        } else if constexpr (BG == kBgGeneral) {
             const double rg = sqrt(rg2);
-            if constexpr (BG == kBgGeneral) {
-                bg += qw / (pow(rg, K.alpha) * pow(rg + K.r0, K.alpha_delta3));
-            } else {
-                const double rs = rg + K.r0;
-                bg += qw / (rg * rs * rs * rs);
-            }
+            bg += qw / (pow(rg, K.alpha) * pow(rg + K.r0, K.alpha_delta3));
+        } else {
+            const double rinv = rsqrt_nr(rg2);
+            const double irs = rcp_nr(fma(rg2, rinv, K.r0));
+            bg = fma(qw * rinv, irs * irs * irs, bg);
         }
     }
 #pragma unroll
     for (int s = 0; s < S; s++) {
-        double v0 = x - K.sc[s][0];
-        double v1 = y - K.sc[s][1];
-        double v2 = z - K.sc[s][2];
-        const double d = K.sa[s][0] * v0 + K.sa[s][1] * v1 + K.sa[s][2] * v2;
-        v0 -= d * K.sa[s][0];
-        v1 -= d * K.sa[s][1];
-        v2 -= d * K.sa[s][2];
-        const double n2 = v0 * v0 + v1 * v1 + v2 * v2;
+        const double w0 = fma(r, v.up[s][0], -K.cperp[s][0]);
+        const double w1 = fma(r, v.up[s][1], -K.cperp[s][1]);
+        const double w2 = fma(r, v.up[s][2], -K.cperp[s][2]);
+        const double a = fma(w0, w0, fma(w1, w1, w2 * w2));
         if constexpr (MIXED) {
-            const float e = __expf((float)(-n2 * K.inv_2s2[s]));
-            st[s] = fma(qw, (double)e, st[s]);
+            // 2^t with t = -a log2(e) split in FP64 into n + f, f in [0, 1):
+            // only 2^f goes through FP32, so the relative error stays at the
+            // FP32 exp2's ulp however large |t| is.
+            const double t = fmax(a * -1.4426950408889634074, -1100.0);
+            const double n = floor(t);
+            const float e = __builtin_amdgcn_exp2f((float)(t - n));
+            st[s] = fma(qw, ldexp((double)e, (int)n), st[s]);
         } else {
-            st[s] += qw * exp(-n2 * K.inv_2s2[s]);
+            st[s] = fma(qw, exp_neg(-a), st[s]);
         }
     }
 }
@@ -178,9 +269,7 @@ __global__ __launch_bounds__(kBlock) void integral_kernel(const RegionLaunch* __
     const long last = (col0 + kBlock < C ? col0 + kBlock : C);
     const bool active = it.calc_end > col0 * R && it.calc_begin < last * R;
     if (active) {
-        const double ux = it.col[cc];
-        const double uy = it.col[C + cc];
-        const double uz = it.col[2 * C + cc];
+        const Sight<S> sight = make_sight<S>(it.col[cc], it.col[C + cc], it.col[2 * C + cc], K);
         const double wnu = it.col[3 * C + cc];
         const long base = cc * (long)R;
         long lo_l = it.calc_begin - base, hi_l = it.calc_end - base;
@@ -206,7 +295,7 @@ __global__ __launch_bounds__(kBlock) void integral_kernel(const RegionLaunch* __
 #pragma unroll 2
                 for (int i = 0; i < n; i++) {
                     const double2 p = row[i];
-                    point_terms<S, BG, MIXED>(p.x, p.y, ux, uy, uz, K, bg, st);
+                    point_terms<S, BG, MIXED>(p.x, p.y, sight, K, bg, st);
                 }
                 const RStep& rs = it.rstep[k];
                 double f = rs.reff_xr_rp3 * rs.irv;
@@ -263,9 +352,7 @@ __global__ __launch_bounds__(kBlock) void star_kernel(const StarLaunch* __restri
     const long i = valid ? idx : N - 1;
     const int n = it.n_conv;
 
-    const double ux = it.star[i];
-    const double uy = it.star[N + i];
-    const double uz = it.star[2 * N + i];
+    const Sight<S> sight = make_sight<S>(it.star[i], it.star[N + i], it.star[2 * N + i], K);
     const double rstar = it.star[3 * N + i];
     const double reff = it.star[4 * N + i];
 
@@ -274,7 +361,7 @@ __global__ __launch_bounds__(kBlock) void star_kernel(const StarLaunch* __restri
     for (int j = 0; j < n; j++) {
         const double r = rstar * conv_ratio[j];
         const double qw = conv_kw[j] * (r * r * r);
-        point_terms<S, BG, MIXED>(r, qw, ux, uy, uz, K, bg, st);
+        point_terms<S, BG, MIXED>(r, qw, sight, K, bg, st);
     }
     if constexpr (BG == kBgZeroQ) {
         bg = -1.0;

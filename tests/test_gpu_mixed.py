@@ -1,0 +1,67 @@
+"""Mixed precision against the FP64 oracle.
+
+The mixed path evaluates the integrand's sqrt, reciprocal and exp in FP32
+and keeps FP64 for positions, the stream-axis distance, all sums and the
+per-star log.  Bar: every star's |delta ln P| <= 1e-7 against the FP64
+oracle; the integrals' relative errors are reported and held to 1e-8.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import fixture, rel
+from milkyway_client_amd import separation as sep
+from milkyway_client_amd import synth
+from milkyway_client_amd.refstructs import ParamsHolder, StarsHolder
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+LNP_TOL = 1e-7        # per star, natural log
+INT_REL_TOL = 1e-8
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    sep.load_library()
+    sep.require_device()
+
+
+def mixed_errors(p, stars, vectors=None):
+    ph, sh = ParamsHolder(p), StarsHolder(stars)
+    vecs = [p.parameter_vector()] if vectors is None else list(vectors)
+    out = []
+    with sep.Context(ph, sh) as ctx:
+        got = ctx.eval(np.array(vecs), precision=sep.PREC_MIXED, per_star=True)
+    for v, g in zip(vecs, got):
+        p.set_parameter_vector(v)
+        ref = O.evaluate_full(ParamsHolder(p), StarsHolder(stars))
+        d_lnp = math.log(10) * np.max(np.abs(g.log10_p - ref.log10_p))
+        out.append(dict(
+            I_bg=rel(g.background_integral, ref.background_integral),
+            I_st=rel(g.stream_integrals, ref.stream_integrals) if p.number_streams else 0.0,
+            lnP=d_lnp, lnL=abs(g.likelihood - ref.likelihood) * math.log(10)))
+    return out
+
+
+@pytest.mark.parametrize("fid", ["79", "86", "11"])
+def test_mixed_reference_fixtures(fid):
+    p, stars = fixture(fid, n_stars=2000)
+    for e in mixed_errors(p, stars):
+        print(fid, {k: f"{v:.2e}" for k, v in e.items()})
+        assert e["lnP"] <= LNP_TOL
+        assert e["I_bg"] <= INT_REL_TOL and e["I_st"] <= INT_REL_TOL
+
+
+def test_mixed_sweep_parameter_vectors():
+    """32 seeded parameter vectors on a coarser grid over the full-size
+    wedge's g, mu and nu spans (70 x 80 x 32 cells, n = 120, 3 streams)."""
+    shape = synth.WedgeShape(70, 80, 32, 120, 3000)
+    p, stars = synth.workload(1, shape, 3)
+    vecs = synth.parameter_vectors(2, p, 32)
+    errs = mixed_errors(p, stars, vecs)
+    worst = {k: max(e[k] for e in errs) for k in errs[0]}
+    print("worst over 32 vectors:", {k: f"{v:.2e}" for k, v in worst.items()})
+    assert worst["lnP"] <= LNP_TOL
+    assert worst["I_bg"] <= INT_REL_TOL and worst["I_st"] <= INT_REL_TOL

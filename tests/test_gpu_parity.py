@@ -187,3 +187,50 @@ def test_repeatable_bitwise():
         b = ctx.eval(p.parameter_vector(), per_star=True)[0]
     assert a.likelihood == b.likelihood
     assert np.array_equal(a.log10_p, b.log10_p)
+
+
+def test_sagittarius_wedges():
+    """sgr_coordinates = 1, with mu ranges that reach Sagittarius longitudes
+    above 360 (the reference's sgrToGal quirk, reproduced by both sides)."""
+    for wedge, mu_lo, mu_hi in [(10, -20.0, 30.0), (82, 80.0, 200.0)]:
+        p, stars = synth.workload(50 + wedge, synth.WedgeShape(20, 30, 12, 20, 400), 2)
+        p.sgr_coordinates, p.wedge = 1, wedge
+        for g in p.regions:
+            g.mu_min, g.mu_max = mu_lo, mu_hi
+        p.streams[0].parameters[0] = 0.5 * (mu_lo + mu_hi)
+        check_against_oracle(p, stars)
+
+
+def test_zero_probability_stars():
+    """P == 0 stars count -238 and num_zero (evaluation_optimized.c:705-716)."""
+    p, stars = synth.workload(60, synth.WedgeShape(20, 24, 8, 15, 400), 1)
+    p.background_weight = -800.0
+    p.streams[0].parameters = [350.0, 40.0, 0.3, 1.0, 0.05]
+    res, ref = check_against_oracle(p, stars)
+    assert ref.num_zero > 0
+
+
+def test_full_size_geometry_slice():
+    """The full-size wedge (700 x 1600 x 640 plus a 700 x 400 x 160 cut,
+    n = 120, 3 streams): the same flat-index slice of every region through
+    the drop-in entry point and through the oracle."""
+    p, stars = synth.workload(1, synth.FULL_SIZE, synth.FULL_SIZE_STREAMS, n_stars=2000)
+    ph, sh, es = holders(p, stars)
+    ph2, sh2, es2 = holders(p, stars)
+    for st in (es, es2):
+        a0 = st.area(0)
+        a0.current_calculation = 700 * 640 * 811 + 12345   # mid-wedge, mid-column
+        a0.max_calculation = a0.current_calculation + 2 * 640 * 700
+        a1 = st.area(1)
+        a1.current_calculation = 700 * 160 * 200
+        a1.max_calculation = a1.current_calculation + 160 * 700
+    sep.calculate_integrals(ph, es, sh)
+    O.integrals(ph2, sh2, es2)
+    for g in range(2):
+        bg, st = es.region_integrals(g)
+        bg2, st2 = es2.region_integrals(g)
+        assert rel(bg, bg2) <= INT_TOL
+        assert rel(st, st2) <= INT_TOL
+    sep.calculate_likelihood(ph, es, sh)
+    O.likelihood(ph2, sh2, es2)
+    assert abs(es.es.prob_sum - es2.es.prob_sum) / len(stars) <= LNL_TOL
