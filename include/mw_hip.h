@@ -135,6 +135,87 @@ size_t mw_ctx_resident_bytes(const mw_ctx* ctx);
 
 void mw_ctx_destroy(mw_ctx* ctx);
 
+/* ------------------------------------------------------------------------
+ * Batched evaluation (no reference counterpart): many parameter vectors over
+ * several resident wedges of uneven size in one persistent launch.  Work
+ * items (one 256-column tile of one region of one vector) are ordered by
+ * cost, largest first, and taken from a device counter by a grid sized to
+ * the GPU; every vector's result is bitwise the same as its single
+ * evaluation.  With MW_BATCH_GRAPH the launch sequence is captured once as a
+ * HIP graph and replayed by later runs with new parameter values.
+ * ------------------------------------------------------------------------ */
+
+typedef struct mw_batch mw_batch;
+
+#define MW_BATCH_GRAPH 1
+
+/* n_vectors[i] parameter vectors for context ctxs[i]; all contexts on one
+ * device.  The contexts must outlive the batch. */
+int mw_batch_create(mw_ctx* const* ctxs, const int* n_vectors, int n_ctxs, int precision,
+                    int flags, mw_batch** out);
+
+/* params: the vectors of ctxs[0], then of ctxs[1], ..., each
+ * mw_ctx_parameter_count(ctx) doubles; results: one per vector, same order. */
+int mw_batch_run(mw_batch* batch, const double* params, mw_result* results);
+
+/* Kernel variant (inner-loop unroll and occupancy target, 0 ..
+ * mw_batch_variant_count() - 1) and persistent blocks per compute unit
+ * (1-16), as chosen by the autotuner. */
+int mw_batch_set_variant(mw_batch* batch, int variant, int blocks_per_cu);
+int mw_batch_variant_count(void);
+int mw_batch_vector_count(const mw_batch* batch);
+/* Timing of the last run: total_ms from the constants' upload to the results'
+ * download; integral_ms the persistent integral launch(es) alone (equal to
+ * total_ms for a replayed graph, where only the whole sequence is timed). */
+int mw_batch_last_timing(const mw_batch* batch, mw_timing* out);
+void mw_batch_destroy(mw_batch* batch);
+
+/* ------------------------------------------------------------------------
+ * Request server (no reference counterpart; the reference evaluates one work
+ * unit per process).  Wedges are made resident once; requests are queued as
+ * they arrive and a dispatcher thread admits them, in arrival order, into
+ * batches whenever a runner (a stream with its own buffers) is free and the
+ * batch's scratch fits the HBM budget next to the resident wedges.  Up to
+ * max_inflight batches run concurrently on the device.  A request's result is
+ * bitwise the same alone and in any mix.  Each request is collected once
+ * (poll or wait).
+ * ------------------------------------------------------------------------ */
+
+typedef struct mw_server mw_server;
+
+typedef struct {
+    size_t hbm_budget_bytes;     /* resident wedges + in-flight scratch; 0 = no limit */
+    int max_inflight;            /* batches on the device at once, 1-16 */
+    int max_batch_vectors;       /* requests admitted per batch */
+    int variant;                 /* kernel variant (see mw_batch_set_variant) */
+    int blocks_per_cu;           /* persistent workgroups per compute unit (0 = 2) */
+} mw_server_config;
+
+typedef struct {
+    double submit_s, dispatch_s, done_s;   /* steady-clock seconds */
+} mw_request_times;
+
+typedef struct {
+    size_t resident_bytes, inflight_scratch_bytes;
+    long pending, batches, completed, largest_batch;
+    int inflight_batches;
+} mw_server_stats;
+
+int mw_server_create(const mw_server_config* cfg, mw_server** out);
+/* Makes a wedge resident (as mw_ctx_create); fails with MW_E_NOMEM beyond the budget. */
+int mw_server_add_wedge(mw_server* server, const ASTRONOMY_PARAMETERS* ap, const STAR_POINTS* sp,
+                        int* wedge_id);
+/* Queues one evaluation (params: mw_ctx_parameter_count doubles); never blocks. */
+int mw_server_submit(mw_server* server, int wedge_id, const double* params, int precision,
+                     long* request_id);
+/* *finished = 1 and the result when the request is done, else 0. */
+int mw_server_poll(mw_server* server, long request_id, mw_result* out, mw_request_times* times,
+                   int* finished);
+int mw_server_wait(mw_server* server, long request_id, mw_result* out, mw_request_times* times);
+int mw_server_get_stats(mw_server* server, mw_server_stats* out);
+/* Finishes queued and in-flight requests, then frees everything. */
+void mw_server_destroy(mw_server* server);
+
 /* Message of the last failure on this thread ("" if none). */
 const char* mw_last_error(void);
 

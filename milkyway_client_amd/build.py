@@ -22,7 +22,7 @@ LIB_NAME = "libmw_separation_hip.so"
 LIB_PATH = LIB_DIR / LIB_NAME
 ARCH = os.environ.get("MW_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["mw_kernels.hip", "mw_api.hip"]
+HIP_SOURCES = ["mw_kernels.hip", "mw_api.hip", "mw_batch.hip", "mw_server.hip"]
 HOST_SOURCES = ["mw_tables.cpp"]
 
 HIPCC_FLAGS = [

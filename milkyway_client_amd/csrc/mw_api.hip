@@ -13,30 +13,11 @@
 #include <string>
 #include <vector>
 
-#include "mw_hip.h"
-#include "mw_internal.h"
-
-namespace mw {
-hipError_t launch_integral(int S, int bg, bool mixed, const RegionLaunch* d_items, int n_items,
-                           int n_blocks, const EvalConst* d_consts, double* d_partials,
-                           hipStream_t stream);
-hipError_t launch_star(int S, int bg, bool mixed, const StarLaunch* d_items, int n_items,
-                       int n_blocks, const EvalConst* d_consts, const double* d_ratio,
-                       const double* d_kw, double* d_partials, double* d_log10p, long stride,
-                       hipStream_t stream);
-hipError_t launch_reduce(bool star, const void* d_items, int n_items, const double* d_partials,
-                         double* d_rows, hipStream_t stream);
-hipError_t launch_finalize(const int* d_first, const int* d_count, const double* d_rows,
-                           EvalConst* d_consts, int n_slots, hipStream_t stream);
-int tile_pairs();
-int integral_comps();
-int star_comps();
-}  // namespace mw
+#include "mw_context.h"
 
 using namespace mw;
 
-namespace {
-
+namespace mw {
 thread_local std::string g_error;
 
 int fail(int code, const std::string& msg)
@@ -44,187 +25,9 @@ int fail(int code, const std::string& msg)
     g_error = msg;
     return code;
 }
+}  // namespace mw
 
-#define MW_HIP_CHECK(expr)                                                                  \
-    do {                                                                                    \
-        hipError_t e_ = (expr);                                                             \
-        if (e_ != hipSuccess)                                                               \
-            return fail(MW_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));       \
-    } while (0)
-
-// Device buffer that grows on demand.
-template <typename T>
-struct DevBuf {
-    T* p = nullptr;
-    size_t cap = 0;
-    ~DevBuf() { release(); }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-    int reserve(size_t n)
-    {
-        if (n <= cap) return MW_OK;
-        release();
-        if (hipMalloc(&p, sizeof(T) * std::max<size_t>(n, 1)) != hipSuccess) {
-            p = nullptr;
-            return fail(MW_E_NOMEM, "hipMalloc failed");
-        }
-        cap = n;
-        return MW_OK;
-    }
-};
-
-// Pinned host buffer that grows on demand.
-template <typename T>
-struct HostBuf {
-    T* p = nullptr;
-    size_t cap = 0;
-    ~HostBuf() { release(); }
-    void release()
-    {
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-    int reserve(size_t n)
-    {
-        if (n <= cap) return MW_OK;
-        release();
-        if (hipHostMalloc((void**)&p, sizeof(T) * std::max<size_t>(n, 1), hipHostMallocDefault) != hipSuccess) {
-            p = nullptr;
-            return fail(MW_E_NOMEM, "hipHostMalloc failed");
-        }
-        cap = n;
-        return MW_OK;
-    }
-};
-
-// Geometry of one region, as INTEGRAL_AREA holds it.
-struct RegionGeom {
-    int r_steps, mu_steps, nu_steps;
-    double r_min, r_max, mu_min, mu_max, nu_min, nu_max;
-    double r_step_size, mu_step_size, nu_step_size;
-    bool operator==(const RegionGeom& o) const { return std::memcmp(this, &o, sizeof(*this)) == 0; }
-};
-
-RegionGeom geom_of(const INTEGRAL_AREA& a)
-{
-    RegionGeom g;
-    std::memset(&g, 0, sizeof(g));
-    g.r_steps = a.r_steps; g.mu_steps = a.mu_steps; g.nu_steps = a.nu_steps;
-    g.r_min = a.r_min; g.r_max = a.r_max; g.mu_min = a.mu_min; g.mu_max = a.mu_max;
-    g.nu_min = a.nu_min; g.nu_max = a.nu_max;
-    g.r_step_size = a.r_step_size; g.mu_step_size = a.mu_step_size; g.nu_step_size = a.nu_step_size;
-    return g;
-}
-
-// INTEGRAL -> INTEGRAL_AREA geometry as initialize_integral_area computes it
-// (evaluation_state.c:97-127).
-RegionGeom geom_of(const INTEGRAL& in)
-{
-    INTEGRAL_AREA a;
-    std::memset(&a, 0, sizeof(a));
-    a.mu_min = in.mu_min; a.mu_max = in.mu_max; a.mu_steps = in.mu_steps;
-    a.nu_min = in.nu_min; a.nu_max = in.nu_max; a.nu_steps = in.nu_steps;
-    a.r_min = in.r_min; a.r_max = in.r_max; a.r_steps = in.r_steps;
-    a.mu_step_size = (a.mu_max - a.mu_min) / a.mu_steps;
-    a.nu_step_size = (a.nu_max - a.nu_min) / a.nu_steps;
-    a.r_step_size = (a.r_max - a.r_min) / a.r_steps;
-    return geom_of(a);
-}
-
-INTEGRAL_AREA area_of(const RegionGeom& g)
-{
-    INTEGRAL_AREA a;
-    std::memset(&a, 0, sizeof(a));
-    a.r_steps = g.r_steps; a.mu_steps = g.mu_steps; a.nu_steps = g.nu_steps;
-    a.r_min = g.r_min; a.r_max = g.r_max; a.mu_min = g.mu_min; a.mu_max = g.mu_max;
-    a.nu_min = g.nu_min; a.nu_max = g.nu_max;
-    a.r_step_size = g.r_step_size; a.mu_step_size = g.mu_step_size; a.nu_step_size = g.nu_step_size;
-    return a;
-}
-
-// Deep copy of the model part of ASTRONOMY_PARAMETERS (values and optimize
-// masks) that parameter vectors are written into.
-struct ModelTemplate {
-    ASTRONOMY_PARAMETERS ap{};
-    std::vector<double> bg;
-    std::vector<int> bg_opt;
-    std::vector<double> sw;
-    std::vector<int> sw_opt;
-    std::vector<std::vector<double>> sp;
-    std::vector<std::vector<int>> sp_opt;
-    std::vector<double*> sp_rows;
-    std::vector<int*> sp_opt_rows;
-
-    void load(const ASTRONOMY_PARAMETERS& src)
-    {
-        ap = src;
-        const int nb = src.number_background_parameters, S = src.number_streams;
-        const int nsp = src.number_stream_parameters;
-        bg.assign(src.background_parameters, src.background_parameters + nb);
-        bg_opt.assign(src.background_optimize, src.background_optimize + nb);
-        sw.assign(src.stream_weights, src.stream_weights + S);
-        sw_opt.assign(src.stream_weight_optimize, src.stream_weight_optimize + S);
-        sp.resize(S);
-        sp_opt.resize(S);
-        sp_rows.resize(S);
-        sp_opt_rows.resize(S);
-        for (int s = 0; s < S; s++) {
-            sp[s].assign(src.stream_parameters[s], src.stream_parameters[s] + nsp);
-            sp_opt[s].assign(src.stream_optimize[s], src.stream_optimize[s] + nsp);
-            sp_rows[s] = sp[s].data();
-            sp_opt_rows[s] = sp_opt[s].data();
-        }
-        ap.background_parameters = bg.data();
-        ap.background_optimize = bg_opt.data();
-        ap.stream_weights = sw.data();
-        ap.stream_weight_optimize = sw_opt.data();
-        ap.stream_parameters = sp_rows.data();
-        ap.stream_optimize = sp_opt_rows.data();
-        ap.background_step = ap.background_min = ap.background_max = nullptr;
-        ap.stream_weight_step = ap.stream_weight_min = ap.stream_weight_max = nullptr;
-        ap.stream_step = ap.stream_min = ap.stream_max = nullptr;
-        ap.integral = nullptr;
-    }
-};
-
-// get_optimized_parameter_count (parameters.c:307-330)
-int optimized_count(const ASTRONOMY_PARAMETERS& ap)
-{
-    int n = 0;
-    for (int i = 0; i < ap.number_background_parameters; i++) n += ap.background_optimize[i] ? 1 : 0;
-    for (int s = 0; s < ap.number_streams; s++) {
-        n += ap.stream_weight_optimize[s] ? 1 : 0;
-        for (int j = 0; j < ap.number_stream_parameters; j++) n += ap.stream_optimize[s][j] ? 1 : 0;
-    }
-    return n;
-}
-
-// set_astronomy_parameters (parameters.c:332-363)
-void apply_vector(ASTRONOMY_PARAMETERS& ap, const double* v)
-{
-    int k = 0;
-    for (int i = 0; i < ap.number_background_parameters; i++)
-        if (ap.background_optimize[i]) ap.background_parameters[i] = v[k++];
-    for (int s = 0; s < ap.number_streams; s++) {
-        if (ap.stream_weight_optimize[s]) ap.stream_weights[s] = v[k++];
-        for (int j = 0; j < ap.number_stream_parameters; j++)
-            if (ap.stream_optimize[s][j]) ap.stream_parameters[s][j] = v[k++];
-    }
-}
-
-struct RegionDev {
-    RegionGeom geom;
-    int n_conv = 0;
-    long n_cols = 0;
-    int blocks = 0;
-    DevBuf<double> rq, col;
-    DevBuf<RStep> rstep;
-};
+namespace {
 
 uint64_t hash_stars(const STAR_POINTS* sp)
 {
@@ -242,51 +45,6 @@ uint64_t hash_stars(const STAR_POINTS* sp)
 }
 
 }  // namespace
-
-struct mw_ctx {
-    int device = 0;
-    int flags = 0;
-    hipStream_t stream = nullptr;
-    int S = 0, n_conv = 0, wedge = 0, sgr = 0;
-    int n_params = 0;
-    ModelTemplate tmpl;
-    std::vector<std::unique_ptr<RegionDev>> regions;
-    long n_stars = 0;
-    DevBuf<double> star, ratio, kw;
-
-    // per-evaluation buffers
-    DevBuf<EvalConst> consts;
-    DevBuf<RegionLaunch> ritems;
-    DevBuf<StarLaunch> sitems;
-    DevBuf<ReduceItem> rred, sred;
-    DevBuf<double> ipart, spart, irows, srows, log10p;
-    DevBuf<int> first, count;
-    HostBuf<EvalConst> h_consts;
-    HostBuf<RegionLaunch> h_ritems;
-    HostBuf<StarLaunch> h_sitems;
-    HostBuf<ReduceItem> h_rred, h_sred;
-    HostBuf<double> h_irows, h_srows;
-    HostBuf<int> h_first, h_count;
-
-    hipEvent_t ev[6] = {};
-    bool have_events = false;
-    mw_timing timing{};
-    std::vector<double> last_regions;   // [n_vec][n_regions][kIntComps]
-    int last_n_vec = 0;
-
-    // drop-in cache key
-    std::vector<RegionGeom> key_geom;
-    const STAR_POINTS* key_sp = nullptr;
-    int key_stars = 0;
-    uint64_t key_hash = 0;
-
-    ~mw_ctx()
-    {
-        if (have_events)
-            for (auto& e : ev) (void)hipEventDestroy(e);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-};
 
 namespace {
 
@@ -404,7 +162,7 @@ int run_plan(mw_ctx* c, Plan& P)
             L.out_index = order[oi];
             L.block_begin = g.n_blocks;          // block index within the group's launch
             L.block_count = rd.blocks;
-            L.pad = 0;
+            L.n_streams = c->S;
             ReduceItem& R = c->h_rred.p[item];
             R.block_begin = total_iblocks;       // global partial index
             R.block_count = rd.blocks;
@@ -448,7 +206,9 @@ int run_plan(mw_ctx* c, Plan& P)
             L.block_begin = g.n_blocks;
             L.block_count = blocks;
             L.out_index = s;
-            L.pad = 0;
+            L.n_streams = c->S;
+            L.ratio = c->ratio.p;
+            L.kw = c->kw.p;
             ReduceItem& R = c->h_sred.p[oi];
             R.block_begin = total_sblocks;
             R.block_count = blocks;

@@ -79,7 +79,7 @@ struct RegionLaunch {
     int slot;                  // evaluation-constant slot
     int out_index;             // index into the per-region result rows
     int block_begin, block_count;
-    int pad;
+    int n_streams;             // streams of the slot's model (batched kernels)
 };
 
 // Stars resident on the device, SoA.
@@ -91,7 +91,9 @@ struct StarLaunch {
     int slot, n_conv;
     int block_begin, block_count;
     int out_index;
-    int pad;
+    int n_streams;             // streams of the slot's model (batched kernel)
+    const double* ratio;       // [n_conv] r_i / r_star of the star's context
+    const double* kw;          // [n_conv] W_i coeff exp(-dx_i^2 / (2 sigma^2))
 };
 
 // Second-pass item: the partials of blocks [block_begin, block_begin +

@@ -240,23 +240,19 @@ __device__ __forceinline__ void point_terms(double r, double qw, const Sight<S>&
 }
 
 // ---------------------------------------------------------------------------
-// Integral kernel
+// Integral: one logical block = 256 consecutive columns of one region of one
+// evaluation slot; the block's double-double partials go to prow.
 // ---------------------------------------------------------------------------
-template <int S, int BG, bool MIXED>
-__global__ __launch_bounds__(kBlock) void integral_kernel(const RegionLaunch* __restrict__ items,
-                                                          int n_items,
-                                                          const EvalConst* __restrict__ consts,
-                                                          double* __restrict__ partials)
+template <int S, int BG, bool MIXED, int UNROLL>
+__device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
+                                               const EvalConst* __restrict__ consts,
+                                               double* __restrict__ prow, double2* tile,
+                                               DD (*red)[kIntComps])
 {
-    __shared__ double2 tile[kTileD2];
-    __shared__ DD red[kBlock / 64][kIntComps];
-
-    const int b = blockIdx.x;
-    const RegionLaunch& it = items[find_item(items, n_items, b)];
     const EvalConst& K = consts[it.slot];
     const int R = it.r_steps, n = it.n_conv;
     const long C = it.n_cols;
-    const long col0 = (long)(b - it.block_begin) * kBlock;
+    const long col0 = lb * kBlock;
     const long col = col0 + threadIdx.x;
     const bool valid = col < C;
     const long cc = valid ? col : C - 1;
@@ -292,7 +288,7 @@ __global__ __launch_bounds__(kBlock) void integral_kernel(const RegionLaunch* __
                 double bg = 0.0;
                 double st[kMaxStreams] = { 0.0, 0.0, 0.0, 0.0 };
                 const double2* row = tile + kk * n;
-#pragma unroll 2
+#pragma unroll UNROLL
                 for (int i = 0; i < n; i++) {
                     const double2 p = row[i];
                     point_terms<S, BG, MIXED>(p.x, p.y, sight, K, bg, st);
@@ -322,32 +318,91 @@ __global__ __launch_bounds__(kBlock) void integral_kernel(const RegionLaunch* __
     }
     block_reduce_dd<kIntComps>(out, red);
     if (threadIdx.x == 0) {
-        double* p = partials + (long)b * kIntComps * 2;
 #pragma unroll
         for (int c = 0; c < kIntComps; c++) {
-            p[2 * c] = out[c].hi;
-            p[2 * c + 1] = out[c].lo;
+            prow[2 * c] = out[c].hi;
+            prow[2 * c + 1] = out[c].lo;
         }
     }
 }
 
+// One hardware block per logical block; the stream count is a template
+// parameter shared by every item of the launch.
+template <int S, int BG, bool MIXED>
+__global__ __launch_bounds__(kBlock) void integral_kernel(const RegionLaunch* __restrict__ items,
+                                                          int n_items,
+                                                          const EvalConst* __restrict__ consts,
+                                                          double* __restrict__ partials)
+{
+    __shared__ double2 tile[kTileD2];
+    __shared__ DD red[kBlock / 64][kIntComps];
+    const int b = blockIdx.x;
+    const RegionLaunch& it = items[find_item(items, n_items, b)];
+    integral_block<S, BG, MIXED, 2>(it, b - it.block_begin, consts,
+                                    partials + (long)b * kIntComps * 2, tile, red);
+}
+
+template <int BG, bool MIXED, int UNROLL>
+__device__ __forceinline__ void integral_block_any(const RegionLaunch& it, long lb,
+                                                   const EvalConst* __restrict__ consts,
+                                                   double* __restrict__ prow, double2* tile,
+                                                   DD (*red)[kIntComps])
+{
+    switch (it.n_streams) {
+    case 0: integral_block<0, BG, MIXED, UNROLL>(it, lb, consts, prow, tile, red); break;
+    case 1: integral_block<1, BG, MIXED, UNROLL>(it, lb, consts, prow, tile, red); break;
+    case 2: integral_block<2, BG, MIXED, UNROLL>(it, lb, consts, prow, tile, red); break;
+    case 3: integral_block<3, BG, MIXED, UNROLL>(it, lb, consts, prow, tile, red); break;
+    default: integral_block<4, BG, MIXED, UNROLL>(it, lb, consts, prow, tile, red); break;
+    }
+}
+
+// Persistent form for batches: a grid sized to the device takes logical
+// blocks from a device counter in order (the host orders items by cost per
+// block, largest first) until every block is taken; every wave then exits.
+// Items of any stream count share the launch.  Results are written per
+// logical block, so they do not depend on which hardware block ran them.
+// WAVES is the amdgpu waves-per-EU occupancy target (1 = compiler default).
+template <int BG, bool MIXED, int UNROLL, int WAVES>
+__global__ __launch_bounds__(kBlock, WAVES) void integral_persistent(
+    const RegionLaunch* __restrict__ items, int n_items, int total_blocks,
+    const EvalConst* __restrict__ consts, double* __restrict__ partials, unsigned* counter)
+{
+    __shared__ double2 tile[kTileD2];
+    __shared__ DD red[kBlock / 64][kIntComps];
+    __shared__ int next;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) next = (int)atomicAdd(counter, 1u);
+        __syncthreads();
+        const int b = next;
+        if (b >= total_blocks) break;
+        const RegionLaunch& it = items[find_item(items, n_items, b)];
+        integral_block_any<BG, MIXED, UNROLL>(it, b - it.block_begin, consts,
+                                              partials + (long)b * kIntComps * 2, tile, red);
+    }
+}
+
+__global__ void reset_counters(unsigned* counters, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) counters[i] = 0u;
+}
+
 // ---------------------------------------------------------------------------
-// Star kernel
+// Star pass
 // ---------------------------------------------------------------------------
 template <int S, int BG, bool MIXED>
-__global__ __launch_bounds__(kBlock) void star_kernel(const StarLaunch* __restrict__ items, int n_items,
-                                                      const EvalConst* __restrict__ consts,
-                                                      const double* __restrict__ conv_ratio,
-                                                      const double* __restrict__ conv_kw,
-                                                      double* __restrict__ partials,
-                                                      double* __restrict__ log10p, long log10p_stride)
+__device__ __forceinline__ void star_block(const StarLaunch& it, long lb,
+                                           const EvalConst* __restrict__ consts,
+                                           const double* __restrict__ conv_ratio,
+                                           const double* __restrict__ conv_kw,
+                                           double* __restrict__ prow, double* __restrict__ log10p,
+                                           long log10p_stride, DD (*red)[kStarComps])
 {
-    __shared__ DD red[kBlock / 64][kStarComps];
-    const int b = blockIdx.x;
-    const StarLaunch& it = items[find_item(items, n_items, b)];
     const EvalConst& K = consts[it.slot];
     const long N = it.n_stars;
-    const long idx = it.star_begin + (long)(b - it.block_begin) * kBlock + threadIdx.x;
+    const long idx = it.star_begin + lb * kBlock + threadIdx.x;
     const bool valid = idx < N;
     const long i = valid ? idx : N - 1;
     const int n = it.n_conv;
@@ -408,12 +463,48 @@ __global__ __launch_bounds__(kBlock) void star_kernel(const StarLaunch* __restri
     }
     block_reduce_dd<kStarComps>(out, red);
     if (threadIdx.x == 0) {
-        double* q = partials + (long)b * kStarComps * 2;
 #pragma unroll
         for (int c = 0; c < kStarComps; c++) {
-            q[2 * c] = out[c].hi;
-            q[2 * c + 1] = out[c].lo;
+            prow[2 * c] = out[c].hi;
+            prow[2 * c + 1] = out[c].lo;
         }
+    }
+}
+
+template <int S, int BG, bool MIXED>
+__global__ __launch_bounds__(kBlock) void star_kernel(const StarLaunch* __restrict__ items, int n_items,
+                                                      const EvalConst* __restrict__ consts,
+                                                      const double* __restrict__ conv_ratio,
+                                                      const double* __restrict__ conv_kw,
+                                                      double* __restrict__ partials,
+                                                      double* __restrict__ log10p, long log10p_stride)
+{
+    __shared__ DD red[kBlock / 64][kStarComps];
+    const int b = blockIdx.x;
+    const StarLaunch& it = items[find_item(items, n_items, b)];
+    star_block<S, BG, MIXED>(it, b - it.block_begin, consts, conv_ratio, conv_kw,
+                             partials + (long)b * kStarComps * 2, log10p, log10p_stride, red);
+}
+
+// Batched star pass: items of any stream count (and any context: each item
+// carries its own star list and convolution tables) in one launch.
+template <int BG, bool MIXED>
+__global__ __launch_bounds__(kBlock) void star_kernel_any(const StarLaunch* __restrict__ items,
+                                                          int n_items,
+                                                          const EvalConst* __restrict__ consts,
+                                                          double* __restrict__ partials)
+{
+    __shared__ DD red[kBlock / 64][kStarComps];
+    const int b = blockIdx.x;
+    const StarLaunch& it = items[find_item(items, n_items, b)];
+    double* prow = partials + (long)b * kStarComps * 2;
+    const long lb = b - it.block_begin;
+    switch (it.n_streams) {
+    case 0: star_block<0, BG, MIXED>(it, lb, consts, it.ratio, it.kw, prow, nullptr, 0, red); break;
+    case 1: star_block<1, BG, MIXED>(it, lb, consts, it.ratio, it.kw, prow, nullptr, 0, red); break;
+    case 2: star_block<2, BG, MIXED>(it, lb, consts, it.ratio, it.kw, prow, nullptr, 0, red); break;
+    case 3: star_block<3, BG, MIXED>(it, lb, consts, it.ratio, it.kw, prow, nullptr, 0, red); break;
+    default: star_block<4, BG, MIXED>(it, lb, consts, it.ratio, it.kw, prow, nullptr, 0, red); break;
     }
 }
 
@@ -560,6 +651,82 @@ hipError_t launch_finalize(const int* d_first, const int* d_count, const double*
     if (n_slots <= 0) return hipSuccess;
     hipLaunchKernelGGL(finalize_kernel, dim3((n_slots + 63) / 64), dim3(64), 0, stream, d_first,
                        d_count, d_rows, d_consts, n_slots);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Batched (persistent) launchers and their tunable variants
+// ---------------------------------------------------------------------------
+// Variant v: inner-loop unroll factor and waves-per-EU occupancy target.
+constexpr int kVariantUnroll[] = { 2, 1, 4, 2, 1, 4 };
+constexpr int kVariantWaves[] = { 1, 1, 1, 5, 5, 5 };
+constexpr int kNumVariants = 6;
+
+using PersistentFn = void (*)(const RegionLaunch*, int, int, const EvalConst*, double*, unsigned*);
+
+template <int BG, bool MIXED, int V>
+PersistentFn persistent_variant()
+{
+    return integral_persistent<BG, MIXED, kVariantUnroll[V], kVariantWaves[V]>;
+}
+
+template <int BG, bool MIXED>
+PersistentFn persistent_fn(int v)
+{
+    switch (v) {
+    case 1: return persistent_variant<BG, MIXED, 1>();
+    case 2: return persistent_variant<BG, MIXED, 2>();
+    case 3: return persistent_variant<BG, MIXED, 3>();
+    case 4: return persistent_variant<BG, MIXED, 4>();
+    case 5: return persistent_variant<BG, MIXED, 5>();
+    default: return persistent_variant<BG, MIXED, 0>();
+    }
+}
+
+// Every variant for the Hernquist profile in both precisions; the other
+// background modes run variant 0 in FP64.
+static PersistentFn pick_persistent(int bg, bool mixed, int v)
+{
+    if (bg == kBgHernquist) return mixed ? persistent_fn<kBgHernquist, true>(v)
+                                         : persistent_fn<kBgHernquist, false>(v);
+    switch (bg) {
+    case kBgHernquistAux: return persistent_variant<kBgHernquistAux, false, 0>();
+    case kBgGeneral: return persistent_variant<kBgGeneral, false, 0>();
+    default: return persistent_variant<kBgZeroQ, false, 0>();
+    }
+}
+
+int persistent_variants() { return kNumVariants; }
+
+hipError_t launch_persistent(int bg, bool mixed, int variant, int grid, const RegionLaunch* d_items,
+                             int n_items, int total_blocks, const EvalConst* d_consts,
+                             double* d_partials, unsigned* d_counter, hipStream_t stream)
+{
+    if (total_blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(pick_persistent(bg, mixed, variant), dim3(grid), dim3(kBlock), 0, stream,
+                       d_items, n_items, total_blocks, d_consts, d_partials, d_counter);
+    return hipGetLastError();
+}
+
+hipError_t launch_reset_counters(unsigned* d_counters, int n, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(reset_counters, dim3((n + 63) / 64), dim3(64), 0, stream, d_counters, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_star_any(int bg, bool mixed, const StarLaunch* d_items, int n_items, int n_blocks,
+                           const EvalConst* d_consts, double* d_partials, hipStream_t stream)
+{
+    if (n_blocks <= 0) return hipSuccess;
+    void (*fn)(const StarLaunch*, int, const EvalConst*, double*);
+    if (mixed && bg == kBgHernquist) fn = star_kernel_any<kBgHernquist, true>;
+    else if (bg == kBgHernquist) fn = star_kernel_any<kBgHernquist, false>;
+    else if (bg == kBgHernquistAux) fn = star_kernel_any<kBgHernquistAux, false>;
+    else if (bg == kBgGeneral) fn = star_kernel_any<kBgGeneral, false>;
+    else fn = star_kernel_any<kBgZeroQ, false>;
+    hipLaunchKernelGGL(fn, dim3(n_blocks), dim3(kBlock), 0, stream, d_items, n_items, d_consts,
+                       d_partials);
     return hipGetLastError();
 }
 

@@ -36,6 +36,10 @@ EXPORTS = [
     "mw_hip_set_precision", "mw_ctx_create", "mw_ctx_parameter_count", "mw_ctx_eval",
     "mw_ctx_region_integrals", "mw_ctx_last_timing", "mw_ctx_resident_bytes",
     "mw_ctx_destroy", "mw_last_error", "mw_device_count", "mw_device_synchronize",
+    "mw_batch_create", "mw_batch_run", "mw_batch_set_variant", "mw_batch_variant_count",
+    "mw_batch_vector_count", "mw_batch_last_timing", "mw_batch_destroy",
+    "mw_server_create", "mw_server_add_wedge", "mw_server_submit", "mw_server_poll",
+    "mw_server_wait", "mw_server_get_stats", "mw_server_destroy",
 ]
 
 
@@ -58,6 +62,21 @@ class mw_result(C.Structure):
 class mw_timing(C.Structure):
     _fields_ = [("integral_ms", C.c_double), ("star_ms", C.c_double), ("total_ms", C.c_double),
                 ("integral_blocks", C.c_long), ("star_blocks", C.c_long)]
+
+
+class mw_server_config(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_size_t), ("max_inflight", C.c_int),
+                ("max_batch_vectors", C.c_int), ("variant", C.c_int), ("blocks_per_cu", C.c_int)]
+
+
+class mw_request_times(C.Structure):
+    _fields_ = [("submit_s", C.c_double), ("dispatch_s", C.c_double), ("done_s", C.c_double)]
+
+
+class mw_server_stats(C.Structure):
+    _fields_ = [("resident_bytes", C.c_size_t), ("inflight_scratch_bytes", C.c_size_t),
+                ("pending", C.c_long), ("batches", C.c_long), ("completed", C.c_long),
+                ("largest_batch", C.c_long), ("inflight_batches", C.c_int)]
 
 
 _lib = None
@@ -102,6 +121,38 @@ def load_library() -> C.CDLL:
     L.mw_device_count.restype = C.c_int
     L.mw_device_synchronize.argtypes = []
     L.mw_device_synchronize.restype = C.c_int
+    L.mw_batch_create.argtypes = [C.POINTER(C.c_void_p), C.POINTER(C.c_int), C.c_int, C.c_int,
+                                  C.c_int, C.POINTER(C.c_void_p)]
+    L.mw_batch_create.restype = C.c_int
+    L.mw_batch_run.argtypes = [C.c_void_p, _dp, C.POINTER(mw_result)]
+    L.mw_batch_run.restype = C.c_int
+    L.mw_batch_set_variant.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.mw_batch_set_variant.restype = C.c_int
+    L.mw_batch_variant_count.argtypes = []
+    L.mw_batch_variant_count.restype = C.c_int
+    L.mw_batch_vector_count.argtypes = [C.c_void_p]
+    L.mw_batch_vector_count.restype = C.c_int
+    L.mw_batch_last_timing.argtypes = [C.c_void_p, C.POINTER(mw_timing)]
+    L.mw_batch_last_timing.restype = C.c_int
+    L.mw_batch_destroy.argtypes = [C.c_void_p]
+    L.mw_batch_destroy.restype = None
+    AP, SP = C.POINTER(ASTRONOMY_PARAMETERS), C.POINTER(STAR_POINTS)
+    L.mw_server_create.argtypes = [C.POINTER(mw_server_config), C.POINTER(C.c_void_p)]
+    L.mw_server_create.restype = C.c_int
+    L.mw_server_add_wedge.argtypes = [C.c_void_p, AP, SP, C.POINTER(C.c_int)]
+    L.mw_server_add_wedge.restype = C.c_int
+    L.mw_server_submit.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, C.POINTER(C.c_long)]
+    L.mw_server_submit.restype = C.c_int
+    L.mw_server_poll.argtypes = [C.c_void_p, C.c_long, C.POINTER(mw_result),
+                                 C.POINTER(mw_request_times), C.POINTER(C.c_int)]
+    L.mw_server_poll.restype = C.c_int
+    L.mw_server_wait.argtypes = [C.c_void_p, C.c_long, C.POINTER(mw_result),
+                                 C.POINTER(mw_request_times)]
+    L.mw_server_wait.restype = C.c_int
+    L.mw_server_get_stats.argtypes = [C.c_void_p, C.POINTER(mw_server_stats)]
+    L.mw_server_get_stats.restype = C.c_int
+    L.mw_server_destroy.argtypes = [C.c_void_p]
+    L.mw_server_destroy.restype = None
     _lib = L
     return L
 
@@ -220,3 +271,129 @@ class Context:
         t = mw_timing()
         _check(load_library().mw_ctx_last_timing(self.handle, C.byref(t)))
         return t
+
+
+BATCH_GRAPH = 1
+
+
+class Batch:
+    """Parameter vectors over several resident wedges in one persistent launch
+    (mw_batch_*).  ``groups`` is a list of (Context, n_vectors)."""
+
+    def __init__(self, groups, precision: int = PREC_MIXED, graph: bool = True):
+        L = load_library()
+        self.contexts = [c for c, _ in groups]
+        self.counts = [int(n) for _, n in groups]
+        arr = (C.c_void_p * len(groups))(*[c.handle.value for c in self.contexts])
+        cnt = (C.c_int * len(groups))(*self.counts)
+        h = C.c_void_p()
+        _check(L.mw_batch_create(arr, cnt, len(groups), precision,
+                                 BATCH_GRAPH if graph else 0, C.byref(h)))
+        self.handle = h
+        self.n_vectors = sum(self.counts)
+
+    @staticmethod
+    def variant_count() -> int:
+        return load_library().mw_batch_variant_count()
+
+    def set_variant(self, variant: int, blocks_per_cu: int):
+        _check(load_library().mw_batch_set_variant(self.handle, variant, blocks_per_cu))
+
+    def run(self, vectors) -> List[Result]:
+        """vectors: one array per context, (n_vectors, n_params) each."""
+        parts = []
+        for ctx, n, v in zip(self.contexts, self.counts, vectors):
+            a = np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(n, max(ctx.n_params, 0)))
+            parts.append(a.ravel())
+        flat = np.ascontiguousarray(np.concatenate(parts) if parts else np.zeros(1))
+        res = (mw_result * self.n_vectors)()
+        _check(load_library().mw_batch_run(self.handle, flat.ctypes.data_as(_dp), res))
+        return [Result(res[i]) for i in range(self.n_vectors)]
+
+    def timing(self) -> mw_timing:
+        t = mw_timing()
+        _check(load_library().mw_batch_last_timing(self.handle, C.byref(t)))
+        return t
+
+    def close(self):
+        if self.handle:
+            load_library().mw_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Server:
+    """Request server (mw_server_*): resident wedges, queued requests admitted
+    into concurrent batches by an HBM budget."""
+
+    def __init__(self, budget_bytes: int = 0, max_inflight: int = 3, max_batch: int = 64,
+                 variant: int = 0, blocks_per_cu: int = 2):
+        cfg = mw_server_config(budget_bytes, max_inflight, max_batch, variant, blocks_per_cu)
+        h = C.c_void_p()
+        _check(load_library().mw_server_create(C.byref(cfg), C.byref(h)))
+        self.handle = h
+        self._keep = []
+        self.n_params = []
+
+    def add_wedge(self, params: ParamsHolder, stars: Optional[StarsHolder]) -> int:
+        wid = C.c_int()
+        _check(load_library().mw_server_add_wedge(self.handle, params.ptr,
+                                                  stars.ptr if stars else None, C.byref(wid)))
+        self._keep.append((params, stars))
+        self.n_params.append(params.py.optimized_parameter_count())
+        return wid.value
+
+    def submit(self, wedge_id: int, vector, precision: int = PREC_MIXED) -> int:
+        v = np.ascontiguousarray(np.asarray(vector, dtype=np.float64).ravel())
+        if v.size == 0:
+            v = np.zeros(1)
+        rid = C.c_long()
+        _check(load_library().mw_server_submit(self.handle, wedge_id, v.ctypes.data_as(_dp),
+                                               precision, C.byref(rid)))
+        return rid.value
+
+    def poll(self, request_id: int):
+        r, t, done = mw_result(), mw_request_times(), C.c_int()
+        _check(load_library().mw_server_poll(self.handle, request_id, C.byref(r), C.byref(t),
+                                             C.byref(done)))
+        if not done.value:
+            return None
+        return Result(r), (t.submit_s, t.dispatch_s, t.done_s)
+
+    def wait(self, request_id: int):
+        r, t = mw_result(), mw_request_times()
+        _check(load_library().mw_server_wait(self.handle, request_id, C.byref(r), C.byref(t)))
+        return Result(r), (t.submit_s, t.dispatch_s, t.done_s)
+
+    def stats(self) -> mw_server_stats:
+        st = mw_server_stats()
+        _check(load_library().mw_server_get_stats(self.handle, C.byref(st)))
+        return st
+
+    def close(self):
+        if self.handle:
+            load_library().mw_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
