@@ -1,15 +1,22 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X separation likelihood evaluation.
 
-A step is one likelihood evaluation (volume integral over every region plus
-the star pass) of one parameter vector on the full-size wedge: main region
-700 r x 1600 mu x 640 nu plus a 700 x 400 x 160 cut, 120 convolution points,
-3 streams, 200k stars, seeded synthetic data.  Inputs (tables, stars) are
-resident in HBM before the timed region; each step uploads its parameter
-vector and reads back its results.
+Headline (default ``--workload batch``): batched mixed-precision likelihood
+evaluations against plain FP64.  A step is one batch of 64 parameter vectors
+(4 streams each, 16 per wedge) over 4 resident wedges of uneven size --
+70x160x64 (n=30, 10k stars), 350x400x160 (n=60, 50k stars), 700x800x320 (n=120,
+100k stars) and 700x1600x640 (n=120, 200k stars), r x mu x nu -- evaluated in
+one persistent launch replayed from a HIP graph, kernel variant chosen by the
+autotuner.  ``value`` is evaluations per second of that path; beside it the
+same 64 vectors evaluated one at a time in plain FP64, the full-size FP64
+evaluation (700x1600x640 plus a 700x400x160 cut, n=120, 3 streams, 200k
+stars), a short Poisson serving run, and the CPU baseline.
 
-Prints one JSON line (the last line of stdout).  With --gpus above 1 it exits:
-this build runs on one GPU.
+``--workload full`` times single full-size evaluations instead (``--precision``
+fp64 or mixed).  Inputs (tables, stars) are resident in HBM before every
+timed region; each step uploads its parameter vectors and reads back its
+results.  Prints one JSON line (the last line of stdout).  --gpus above 1
+exits: this build runs on one GPU.
 """
 from __future__ import annotations
 
@@ -21,159 +28,326 @@ import time
 
 import numpy as np
 
+from milkyway_client_amd import autotune, serving, synth, workmodel
 from milkyway_client_amd import separation as sep
-from milkyway_client_amd import synth, workmodel
 from milkyway_client_amd.refstructs import ParamsHolder, StarsHolder, StateHolder
 
-FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 vector (and matrix) peak
-FP32_PEAK_TFLOPS = 157.3    # MI355X FP32 vector peak (packed)
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector peak
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector peak (what the mixed path is priced against)
+BATCH_STREAMS = 4
+PROFILES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
 
-WORKLOADS = {
-    "full": (synth.FULL_SIZE, synth.FULL_SIZE_STREAMS, 1),
-    "reference": (synth.REFERENCE_SIZE, 1, 0),
-}
+
+def measured_traffic(key: str):
+    """HBM bytes per launch of a kernel/workload, as measured by the
+    rocprofv3 FETCH_SIZE + WRITE_SIZE runs summarised under profiles/ (a
+    'traffic_bytes_per_launch: <key> <bytes>' line), or None."""
+    try:
+        for name in sorted(os.listdir(PROFILES)):
+            with open(os.path.join(PROFILES, name)) as f:
+                for line in f:
+                    parts = line.split()
+                    if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch:" and parts[1] == key:
+                        return int(parts[2])
+    except OSError:
+        pass
+    return None
+BATCH_VECTORS_PER_WEDGE = 16
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(p, stars, rows_main: int, rows_cut: int, n_star_sample: int):
-    """Times the oracle (a C restatement of the reference's FP64 CPU path) on
-    one host core over a bounded sample of the same workload: the first
-    rows_main mu-rows of the main region, rows_cut of each cut, and the
-    first n_star_sample stars; scales each part linearly to the full
-    evaluation (work per mu-row and per star is uniform)."""
+# --------------------------------------------------------------------------
+# CPU baseline: the oracle on one core over a bounded sample, scaled
+# --------------------------------------------------------------------------
+
+def cpu_seconds_per_eval(p, stars, rows_per_region, n_star_sample):
+    """Seconds of one full evaluation of (p, stars) by the oracle on one
+    core, from a sample of rows_per_region mu-rows per region and
+    n_star_sample stars (cost per mu-row and per star is uniform)."""
     from oracle import oracle as O
     O.build()
     ph = ParamsHolder(p)
     sample = stars[:n_star_sample]
     sh = StarsHolder(sample)
     st = StateHolder(ph, sh)
-    full_scale = 0.0
-    t0 = time.perf_counter()
     for g in range(ph.ap.number_integrals):
         a = st.area(g)
-        rows = rows_main if g == 0 else rows_cut
-        a.max_calculation = rows * a.nu_steps * a.r_steps
+        a.max_calculation = min(rows_per_region, a.mu_steps) * a.nu_steps * a.r_steps
+    t0 = time.perf_counter()
     O.integrals(ph, sh, st)
     t_int = time.perf_counter() - t0
-    # every region's sample is rows/mu_steps of it; scale region by region
-    # using the sampled points (regions have equal cost per point)
-    sampled_pts = sum((rows_main if g == 0 else rows_cut) * r.nu_steps * r.r_steps
-                      for g, r in enumerate(p.regions))
-    total_pts = sum(r.mu_steps * r.nu_steps * r.r_steps for r in p.regions)
-    full_scale = total_pts / sampled_pts
+    sampled = sum(min(rows_per_region, r.mu_steps) * r.nu_steps * r.r_steps for r in p.regions)
+    total = sum(r.mu_steps * r.nu_steps * r.r_steps for r in p.regions)
     st.es.background_integral = 1.0
     for s in range(p.number_streams):
         st.es.stream_integrals[s] = 1.0
     t0 = time.perf_counter()
     O.likelihood(ph, sh, st)
     t_star = time.perf_counter() - t0
-    t_eval = t_int * full_scale + t_star * (len(stars) / len(sample))
-    return {
-        "value": 1.0 / t_eval, "unit": "evals/s", "cores": 1, "kind": "port",
-        "sample": (f"oracle FP64, 1 thread: {rows_main} of {p.regions[0].mu_steps} main mu-rows"
-                   + (f" + {rows_cut} of {p.regions[1].mu_steps} cut mu-rows" if len(p.regions) > 1 else "")
-                   + f" ({t_int:.1f} s) and {len(sample)} of {len(stars)} stars ({t_star:.1f} s),"
-                   f" scaled linearly to one full evaluation = {t_eval:.0f} s"),
-        "seconds_per_eval": t_eval,
+    return t_int * total / sampled + t_star * len(stars) / len(sample), t_int + t_star
+
+
+# --------------------------------------------------------------------------
+# Workloads
+# --------------------------------------------------------------------------
+
+def build_batch():
+    groups, vecs, plist = [], [], []
+    for k, shape in enumerate(synth.BATCH_SHAPES):
+        p, stars = synth.workload(100 + k, shape, BATCH_STREAMS)
+        ctx = sep.Context(ParamsHolder(p), StarsHolder(stars))
+        desc = (f"{shape.r_steps}x{shape.mu_steps}x{shape.nu_steps}n{shape.convolve}"
+                f"s{BATCH_STREAMS}N{shape.n_stars}")
+        groups.append((ctx, BATCH_VECTORS_PER_WEDGE, desc))
+        vecs.append(synth.parameter_vectors(100 + k, p, BATCH_VECTORS_PER_WEDGE))
+        plist.append((p, stars))
+    return groups, vecs, plist
+
+
+def full_size_fp64(steps: int, warmup: int, precision=sep.PREC_FP64):
+    p, stars = synth.workload(1, synth.FULL_SIZE, synth.FULL_SIZE_STREAMS)
+    t0 = time.perf_counter()
+    ctx = sep.Context(ParamsHolder(p), StarsHolder(stars), timing=True)
+    setup = time.perf_counter() - t0
+    vecs = synth.parameter_vectors(1, p, 4)
+    for i in range(warmup):
+        ctx.eval(vecs[i % 4], precision=precision)
+    lat, kern, star = [], [], []
+    t0 = time.perf_counter()
+    for i in range(steps):
+        s0 = time.perf_counter()
+        ctx.eval(vecs[i % 4], precision=precision)
+        lat.append(time.perf_counter() - s0)
+        tm = ctx.timing()
+        kern.append(tm.integral_ms)
+        star.append(tm.star_ms)
+    el = time.perf_counter() - t0
+    ctx.close()
+    flops = workmodel.integral_flops(p.regions, p.convolve, synth.FULL_SIZE_STREAMS)
+    k_ms = float(np.mean(kern))
+    return p, stars, {
+        "value": steps / el, "unit": "evals/s", "ms_per_eval": 1e3 * el / steps,
+        "integral_kernel_ms": round(k_ms, 3), "star_kernel_ms": round(float(np.mean(star)), 4),
+        "achieved_tflops": round(flops / (k_ms * 1e-3) / 1e12, 3),
+        "frac_fp64_peak": round(flops / (k_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
+        "resident_setup_s": round(setup, 3),
+        "workload": "700x1600x640 + 700x400x160 cut (r x mu x nu), n=120, 3 streams, 200k stars",
     }
+
+
+def serving_run(n_requests: int, rate: float, variant: int, bpc: int):
+    wedges = serving.make_wedges(8, seed=4000)
+    holders = [(ParamsHolder(p), StarsHolder(s)) for p, s, _ in wedges]
+    reqs = serving.make_requests(wedges, n_requests, seed=5000)
+    arr = serving.poisson_arrivals(len(reqs), rate, seed=6000)
+    out = {}
+    with sep.Server(max_inflight=3, max_batch=32, variant=variant, blocks_per_cu=bpc) as srv:
+        ids = [srv.add_wedge(ph, sh) for ph, sh in holders]
+        served = serving.run(srv, ids, reqs, arr)
+        st = srv.stats()
+    with sep.Server(max_inflight=1, max_batch=1) as srv:
+        ids = [srv.add_wedge(ph, sh) for ph, sh in holders]
+        plain = serving.run(srv, ids, reqs, arr, force_precision=sep.PREC_FP64)
+    for k, v in (("continuous_batching_mixed", served), ("one_at_a_time_fp64", plain)):
+        out[k] = {"throughput_req_per_s": round(v["throughput"], 3),
+                  "p50_ms": round(v["p50_ms"], 2), "p99_ms": round(v["p99_ms"], 2)}
+    out["requests"] = n_requests
+    out["poisson_rate_per_s"] = rate
+    out["batches"] = int(st.batches)
+    out["largest_batch"] = int(st.largest_batch)
+    out["wedges"] = "8 resident wedges from the batch shapes, 1-4 streams, 10% FP64 requests"
+    return out
+
+
+# --------------------------------------------------------------------------
+
+def bench_batch(args):
+    t0 = time.perf_counter()
+    groups, vecs, plist = build_batch()
+    log(f"batch workload resident in {time.perf_counter() - t0:.2f} s")
+    tune_t0 = time.perf_counter()
+    choice = autotune.tune(groups, vecs, sep.PREC_MIXED, proxy_vectors=1, reps=1,
+                           blocks_per_cu=(2, 4), use_cache=False, log=log)
+    tune_s = time.perf_counter() - tune_t0
+    log(f"autotuner: variant {choice['variant']} blocks/CU {choice['blocks_per_cu']} ({tune_s:.1f} s)")
+    pairs = [(c, n) for c, n, _ in groups]
+    n_vec = sum(n for _, n in pairs)
+
+    # Roofline: one eager run times the persistent integral launch alone.
+    with sep.Batch(pairs, precision=sep.PREC_MIXED, graph=False) as eager:
+        eager.set_variant(choice["variant"], choice["blocks_per_cu"])
+        eager.run(vecs)
+        eager.run(vecs)
+        tm = eager.timing()
+    int_ms = tm.integral_ms
+    flops = sum(workmodel.integral_flops(p.regions, p.convolve, BATCH_STREAMS) * BATCH_VECTORS_PER_WEDGE
+                for p, _ in plist)
+    bytes_alg = sum(workmodel.integral_bytes(p.regions, p.convolve) * BATCH_VECTORS_PER_WEDGE
+                    for p, _ in plist)
+    log(f"eager batch: persistent integral launch {int_ms:.1f} ms of {tm.total_ms:.1f} ms")
+
+    with sep.Batch(pairs, precision=sep.PREC_MIXED, graph=True) as b:
+        b.set_variant(choice["variant"], choice["blocks_per_cu"])
+        for i in range(args.warmup):
+            b.run(vecs)
+        sep.synchronize()
+        lat = []
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            s0 = time.perf_counter()
+            res = b.run(vecs)
+            lat.append(time.perf_counter() - s0)
+            log(f"step {i}: batch of {n_vec} in {lat[-1] * 1e3:.1f} ms (device {b.timing().total_ms:.1f} ms)")
+        sep.synchronize()
+        elapsed = time.perf_counter() - t0
+    value = args.steps * n_vec / elapsed
+
+    # Plain FP64, one vector at a time, same vectors.
+    t0 = time.perf_counter()
+    for (ctx, n, _), v in zip(groups, vecs):
+        for row in v:
+            ctx.eval(row, precision=sep.PREC_FP64)
+    sep.synchronize()
+    plain_s = time.perf_counter() - t0
+    plain = n_vec / plain_s
+    log(f"plain FP64 one at a time: {plain:.3f} evals/s")
+
+    # Mixed precision one at a time (batching's own share of the gain).
+    t0 = time.perf_counter()
+    for (ctx, n, _), v in zip(groups, vecs):
+        for row in v[:4]:
+            ctx.eval(row, precision=sep.PREC_MIXED)
+    mixed_single = 4 * len(groups) / (time.perf_counter() - t0)
+
+    out = {
+        "metric": "likelihood evals/s, batched mixed precision vs plain FP64",
+        "value": round(value, 4), "unit": "evals/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64+f32",
+        "data": "synthetic (seeded wedge geometries, parameter vectors and stars)",
+        "config": {
+            "workload": ("batch: 64 parameter vectors (4 streams, 16 per wedge) over 4 wedges "
+                         "70x160x64 n=30 10k stars, 350x400x160 n=60 50k, 700x800x320 n=120 100k, "
+                         "700x1600x640 n=120 200k (r x mu x nu); one persistent launch "
+                         "replayed from a HIP graph"),
+            "batch": n_vec, "precision": "mixed",
+            "kernel_variant": choice["variant"], "blocks_per_cu": choice["blocks_per_cu"],
+            "parallelism": "single GPU",
+        },
+        "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 2),
+                       "p99": round(1e3 * float(np.percentile(lat, 99)), 2),
+                       "what": "one batch of 64 evaluations"},
+        "plain_fp64": {"value": round(plain, 4), "unit": "evals/s",
+                       "ratio_batched_mixed_over_plain_fp64": round(value / plain, 3),
+                       "what": "the same 64 vectors, one evaluation at a time, FP64"},
+        "single_mixed": {"value": round(mixed_single, 4), "unit": "evals/s",
+                         "what": "16 of the vectors one at a time, mixed precision"},
+        "roofline": {
+            "bound": "valu", "kernel": "integral_persistent (mixed)",
+            "achieved": round(flops / (int_ms * 1e-3) / 1e12, 3), "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(flops / (int_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
+            "traffic": measured_traffic("integral_persistent_mixed_batch64"),
+            "algorithmic_flop_eq_per_launch": flops,
+            "algorithmic_bytes_per_launch": bytes_alg,
+            "avg_launch_ms": round(int_ms, 3),
+        },
+        "tuning": {"seconds": round(tune_s, 2),
+                   "table": choice["table"]},
+    }
+    for ctx, *_ in groups:
+        ctx.close()
+
+    if not args.no_full:
+        log("full-size FP64 evaluation")
+        _, _, full = full_size_fp64(3, 1)
+        out["full_size_fp64"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in full.items()}
+    if not args.no_serving:
+        log("serving run")
+        out["serving"] = serving_run(args.serving_requests, args.serving_rate, choice["variant"],
+                                     choice["blocks_per_cu"])
+    if not args.no_cpu:
+        log("CPU baseline (oracle, 1 thread, bounded sample)")
+        total_s, spent = 0.0, 0.0
+        for p, stars in plist:
+            s, used = cpu_seconds_per_eval(p, stars, 3, 2000)
+            total_s += s * BATCH_VECTORS_PER_WEDGE
+            spent += used
+        out["cpu_baseline"] = {
+            "value": round(n_vec / total_s, 8), "unit": "evals/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle FP64 (C restatement of the reference CPU path), 1 thread: three mu-rows of "
+                       f"every region of each of the 4 wedges and 2000 stars each ({spent:.1f} s), "
+                       f"scaled linearly to the 64-vector batch = {total_s:.0f} s"),
+        }
+    else:
+        out["cpu_baseline"] = None
+    return out
+
+
+def bench_full(args):
+    prec = sep.PREC_MIXED if args.precision == "mixed" else sep.PREC_FP64
+    p, stars, full = full_size_fp64(args.steps, args.warmup, prec)
+    flops = workmodel.integral_flops(p.regions, p.convolve, synth.FULL_SIZE_STREAMS)
+    out = {
+        "metric": "likelihood evals/s (separation integral + star pass)",
+        "value": round(full["value"], 4), "unit": "evals/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(full["ms_per_eval"], 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if prec == sep.PREC_FP64 else "f64+f32",
+        "data": "synthetic (seeded wedge geometry, parameters and stars)",
+        "config": {"workload": "full: " + full["workload"], "precision": args.precision,
+                   "batch": 1, "parallelism": "single GPU"},
+        "star_kernel_ms": full["star_kernel_ms"],
+        "resident_setup_s": full["resident_setup_s"],
+        "roofline": {"bound": "valu", "kernel": "integral_kernel", "achieved": full["achieved_tflops"],
+                     "peak": FP64_PEAK_TFLOPS if prec == sep.PREC_FP64 else FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s",
+                     "frac": round(full["achieved_tflops"] / (FP64_PEAK_TFLOPS if prec == sep.PREC_FP64
+                                                              else FP32_PEAK_TFLOPS), 4),
+                     "traffic": (measured_traffic("integral_kernel_fp64_full")
+                                 if prec == sep.PREC_FP64 else None),
+                     "algorithmic_flop_eq_per_launch": flops,
+                     "avg_launch_ms": full["integral_kernel_ms"]},
+    }
+    if not args.no_cpu:
+        s, spent = cpu_seconds_per_eval(p, stars, 4, 4000)
+        out["cpu_baseline"] = {
+            "value": round(1.0 / s, 8), "unit": "evals/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle FP64, 1 thread: 4 mu-rows of every region and 4000 stars ({spent:.1f} s), "
+                       f"scaled linearly to one full evaluation = {s:.0f} s")}
+    else:
+        out["cpu_baseline"] = None
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="full")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
-    ap.add_argument("--batch", type=int, default=1, help="parameter vectors per step")
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["batch", "full"], default="batch")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+                    help="precision of --workload full")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-rows", type=int, default=4)
-    ap.add_argument("--cpu-stars", type=int, default=4000)
+    ap.add_argument("--no-full", action="store_true", help="skip the full-size FP64 line")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving run")
+    ap.add_argument("--serving-requests", type=int, default=40)
+    ap.add_argument("--serving-rate", type=float, default=10.0)
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py: this build evaluates on one GPU only; --gpus must be 1", file=sys.stderr)
         sys.exit(2)
-
     sep.load_library()
     sep.require_device()
-    shape, n_streams, k = WORKLOADS[args.workload]
-    t0 = time.perf_counter()
-    p, stars = synth.workload(k, shape, n_streams)
-    ph, sh = ParamsHolder(p), StarsHolder(stars)
-    t_gen = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    ctx = sep.Context(ph, sh, timing=True)
-    t_setup = time.perf_counter() - t0
-    log(f"workload {args.workload}: generated in {t_gen:.1f} s, resident setup {t_setup:.2f} s, "
-        f"{ctx.resident_bytes / 2**20:.1f} MiB on device")
-    prec = sep.PREC_MIXED if args.precision == "mixed" else sep.PREC_FP64
-    vecs = synth.parameter_vectors(k, p, 4 * args.batch)
-
-    def step(i):
-        j = (i % 4) * args.batch
-        return ctx.eval(vecs[j:j + args.batch], precision=prec)
-
-    for i in range(args.warmup):
-        step(i)
-    sep.synchronize()
-    lat, t_int, t_star = [], [], []
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        s0 = time.perf_counter()
-        res = step(i)
-        lat.append(time.perf_counter() - s0)
-        tm = ctx.timing()
-        t_int.append(tm.integral_ms)
-        t_star.append(tm.star_ms)
-        log(f"step {i}: {lat[-1] * 1e3:.2f} ms (integral kernel {tm.integral_ms:.2f} ms, "
-            f"star kernel {tm.star_ms:.2f} ms) lnL-3={res[0].likelihood - 3.0:.12f}")
-    sep.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = 1e3 * elapsed / args.steps
-    evals_per_s = args.steps * args.batch / elapsed
-
-    int_ms = float(np.mean(t_int))
-    flops = workmodel.integral_flops(p.regions, p.convolve, n_streams) * args.batch
-    peak = FP64_PEAK_TFLOPS if prec == sep.PREC_FP64 else FP32_PEAK_TFLOPS
-    achieved = flops / (int_ms * 1e-3) / 1e12
-    roofline = {
-        "bound": "valu", "kernel": "integral_kernel", "achieved": round(achieved, 3),
-        "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
-        "algorithmic_flop_eq_per_launch": flops,
-        "avg_launch_ms": round(int_ms, 4),
-    }
-    out = {
-        "metric": "likelihood evals/s (separation integral + star pass)",
-        "value": round(evals_per_s, 4), "unit": "evals/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if prec == sep.PREC_FP64 else "f32+f64",
-        "data": "synthetic (seeded wedge geometry, parameters and stars)",
-        "config": {
-            "workload": (f"{args.workload}: {' + '.join(f'{g.r_steps}x{g.mu_steps}x{g.nu_steps}' for g in p.regions)}"
-                         f" (r x mu x nu), n_convolve={p.convolve}, streams={n_streams},"
-                         f" stars={len(stars)}"),
-            "precision": args.precision, "batch": args.batch,
-            "parallelism": "single GPU",
-        },
-        "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 3),
-                       "p99": round(1e3 * float(np.percentile(lat, 99)), 3)},
-        "star_kernel_ms": round(float(np.mean(t_star)), 4),
-        "resident_setup_s": round(t_setup, 3),
-        "roofline": roofline,
-    }
-    if not args.no_cpu:
-        cb = cpu_baseline(p, stars, args.cpu_rows, 1, args.cpu_stars)
-        out["cpu_baseline"] = {k2: cb[k2] for k2 in ("value", "unit", "cores", "kind", "sample")}
-        out["cpu_baseline"]["value"] = round(cb["value"], 8)
-    else:
-        out["cpu_baseline"] = None
-    ctx.close()
+    out = bench_batch(args) if args.workload == "batch" else bench_full(args)
     print(json.dumps(out), flush=True)
 
 

@@ -70,6 +70,8 @@ def build(verbose: bool = False) -> Path:
     tmp = LIB_DIR / (LIB_NAME + ".tmp")
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", *objs, "-o", str(tmp)])
     os.replace(tmp, LIB_PATH)
+    # Byte-counter calibration probe (a plain copy kernel of known size).
+    _run([hipcc, *HIPCC_FLAGS, str(CSRC / "hbm_copy_probe.hip"), "-o", str(LIB_DIR / "hbm_copy_probe")])
     return LIB_PATH
 
 
