@@ -58,8 +58,9 @@ struct alignas(64) EvalConst {
     double on[kMaxStreams];
     // Stream-axis distance in linear form.  With c' = c + (8.5, 0, 0) the
     // offset of a point r u - (8.5, 0, 0) from the axis is r u_perp - c_perp,
-    // where x_perp = x - (x . a) a.  Both are scaled by k = 1 / (sigma sqrt 2),
-    // so |r u_perp k - c_perp k|^2 is the exponent |v_perp|^2 / (2 sigma^2).
+    // where x_perp = x - (x . a) a.  Both are scaled by
+    // k = sqrt(log2(e) / (2 sigma^2)), so |r u_perp k - c_perp k|^2 is the
+    // exponent |v_perp|^2 / (2 sigma^2) in base 2.
     double ks[kMaxStreams];
     double cperp[kMaxStreams][3];
     // likelihood weights: exp(w_b), exp(w_s), 0.001 * (exp(w_b) + sum exp(w_s))

@@ -442,7 +442,7 @@ bool make_eval_const(const ASTRONOMY_PARAMETERS& ap, EvalConst* k, std::string* 
         const bool off = sigma > -0.0001 && sigma < 0.0001;
         k->on[s] = off ? 0.0 : 1.0;
         k->inv_2s2[s] = off ? 0.0 : 1.0 / (2.0 * sigma * sigma);
-        k->ks[s] = std::sqrt(k->inv_2s2[s]);
+        k->ks[s] = std::sqrt(k->inv_2s2[s] * 1.4426950408889634074);   // log2(e)
         const double cp[3] = { k->sc[s][0] + kSunR0, k->sc[s][1], k->sc[s][2] };
         const double d = cp[0] * k->sa[s][0] + cp[1] * k->sa[s][1] + cp[2] * k->sa[s][2];
         for (int j = 0; j < 3; j++) k->cperp[s][j] = (cp[j] - d * k->sa[s][j]) * k->ks[s];
