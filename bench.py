@@ -275,12 +275,12 @@ def bench_batch(args):
         log("CPU baseline (oracle, 1 thread, bounded sample)")
         total_s, spent = 0.0, 0.0
         for p, stars in plist:
-            s, used = cpu_seconds_per_eval(p, stars, 3, 2000)
+            s, used = cpu_seconds_per_eval(p, stars, 6, 2000)
             total_s += s * BATCH_VECTORS_PER_WEDGE
             spent += used
         out["cpu_baseline"] = {
             "value": round(n_vec / total_s, 8), "unit": "evals/s", "cores": 1, "kind": "port",
-            "sample": (f"oracle FP64 (C restatement of the reference CPU path), 1 thread: three mu-rows of "
+            "sample": (f"oracle FP64 (C restatement of the reference CPU path), 1 thread: six mu-rows of "
                        f"every region of each of the 4 wedges and 2000 stars each ({spent:.1f} s), "
                        f"scaled linearly to the 64-vector batch = {total_s:.0f} s"),
         }
