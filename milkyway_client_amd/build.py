@@ -24,6 +24,10 @@ ARCH = os.environ.get("MW_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["mw_kernels.hip", "mw_api.hip", "mw_batch.hip", "mw_server.hip"]
 HOST_SOURCES = ["mw_tables.cpp"]
+C_SOURCES = ["mw_io.c"]
+CLIENT_SOURCE = "client/mw_client.c"
+CLIENT_NAME = "milkyway_separation_hip"
+CLIENT_PATH = LIB_DIR / CLIENT_NAME
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
@@ -32,6 +36,7 @@ HIPCC_FLAGS = [
 ]
 GXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall",
              "-pthread"]
+GCC_FLAGS = ["-O2", "-std=c99", "-D_GNU_SOURCE", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter"]
 
 
 def _hipcc() -> str:
@@ -62,14 +67,23 @@ def build(verbose: bool = False) -> Path:
     for src in HOST_SOURCES:
         obj = obj_dir / (Path(src).stem + ".o")
         jobs.append(["g++", *GXX_FLAGS, *inc, "-c", str(CSRC / src), "-o", str(obj)])
+    for src in C_SOURCES:
+        obj = obj_dir / (Path(src).stem + ".o")
+        jobs.append(["gcc", *GCC_FLAGS, *inc, "-c", str(CSRC / src), "-o", str(obj)])
     with ThreadPoolExecutor(max_workers=min(4, len(jobs))) as ex:
         for r in ex.map(_run, jobs):
             if verbose and (r.stdout or r.stderr):
                 print(r.stdout + r.stderr, file=sys.stderr)
-    objs = [str(obj_dir / (Path(s).stem + ".o")) for s in HIP_SOURCES + HOST_SOURCES]
+    objs = [str(obj_dir / (Path(s).stem + ".o")) for s in HIP_SOURCES + HOST_SOURCES + C_SOURCES]
     tmp = LIB_DIR / (LIB_NAME + ".tmp")
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", *objs, "-o", str(tmp)])
     os.replace(tmp, LIB_PATH)
+    # Command-line client, linked against the library with an $ORIGIN rpath
+    # so the pair can move together.
+    tmp = LIB_DIR / (CLIENT_NAME + ".tmp")
+    _run(["gcc", *GCC_FLAGS, *inc, str(CSRC / CLIENT_SOURCE), "-o", str(tmp), f"-L{LIB_DIR}",
+          "-lmw_separation_hip", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, CLIENT_PATH)
     # Byte-counter calibration probe (a plain copy kernel of known size).
     _run([hipcc, *HIPCC_FLAGS, str(CSRC / "hbm_copy_probe.hip"), "-o", str(LIB_DIR / "hbm_copy_probe")])
     return LIB_PATH

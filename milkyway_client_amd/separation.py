@@ -24,6 +24,8 @@ from .refstructs import (ASTRONOMY_PARAMETERS, EVALUATION_STATE, STAR_POINTS, Pa
                          StarsHolder, StateHolder)
 
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libmw_separation_hip.so"
+# The command-line client (src/milkyway_client_app.c's worker on this library).
+CLIENT_PATH = LIB_PATH.parent / "milkyway_separation_hip"
 
 MW_OK, MW_E_INVALID, MW_E_HIP, MW_E_NOMEM, MW_E_NODEVICE = 0, 1, 2, 3, 4
 PREC_FP64, PREC_MIXED = 0, 1
@@ -40,6 +42,10 @@ EXPORTS = [
     "mw_batch_vector_count", "mw_batch_last_timing", "mw_batch_destroy",
     "mw_server_create", "mw_server_add_wedge", "mw_server_submit", "mw_server_poll",
     "mw_server_wait", "mw_server_get_stats", "mw_server_destroy",
+    "mw_read_astronomy_parameters", "mw_free_astronomy_parameters", "mw_read_star_points",
+    "mw_free_star_points", "mw_initialize_state", "mw_free_state",
+    "mw_get_optimized_parameter_count", "mw_set_astronomy_parameters",
+    "mw_write_astronomy_parameters", "mw_write_star_points",
 ]
 
 
@@ -153,6 +159,27 @@ def load_library() -> C.CDLL:
     L.mw_server_get_stats.restype = C.c_int
     L.mw_server_destroy.argtypes = [C.c_void_p]
     L.mw_server_destroy.restype = None
+    ES = C.POINTER(EVALUATION_STATE)
+    L.mw_read_astronomy_parameters.argtypes = [C.c_char_p, AP]
+    L.mw_read_astronomy_parameters.restype = C.c_int
+    L.mw_free_astronomy_parameters.argtypes = [AP]
+    L.mw_free_astronomy_parameters.restype = None
+    L.mw_read_star_points.argtypes = [C.c_char_p, SP]
+    L.mw_read_star_points.restype = C.c_int
+    L.mw_free_star_points.argtypes = [SP]
+    L.mw_free_star_points.restype = None
+    L.mw_initialize_state.argtypes = [AP, SP, ES]
+    L.mw_initialize_state.restype = C.c_int
+    L.mw_free_state.argtypes = [ES]
+    L.mw_free_state.restype = None
+    L.mw_get_optimized_parameter_count.argtypes = [AP]
+    L.mw_get_optimized_parameter_count.restype = C.c_int
+    L.mw_set_astronomy_parameters.argtypes = [AP, _dp]
+    L.mw_set_astronomy_parameters.restype = None
+    L.mw_write_astronomy_parameters.argtypes = [C.c_char_p, AP]
+    L.mw_write_astronomy_parameters.restype = C.c_int
+    L.mw_write_star_points.argtypes = [C.c_char_p, SP]
+    L.mw_write_star_points.restype = C.c_int
     _lib = L
     return L
 
