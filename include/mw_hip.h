@@ -216,6 +216,50 @@ int mw_server_get_stats(mw_server* server, mw_server_stats* out);
 /* Finishes queued and in-flight requests, then frees everything. */
 void mw_server_destroy(mw_server* server);
 
+/* ------------------------------------------------------------------------
+ * Stream separation (milkyway_separation/cpu/separation.c:37-254)
+ *
+ * After an evaluation, each star gets each stream's membership probability
+ *   p_s = (eps_s S_s / I_s) / (sum_t eps_t S_t / I_t + eps_b B / I_b),
+ *   eps_s = exp(w_s) / (1 + sum_t exp(w_t)),  eps_b = 1 / (1 + sum_t exp(w_t)),
+ * with B and S_s the star's convolved densities and I the integrals.  It is
+ * then assigned to stream s (1-based) or to the background (0) by the i-th
+ * drand48() number after srand48(seed) (prob_ok, probability.c:310-403).
+ * The reference seeds with time(NULL); a caller that wants the reference's
+ * behaviour passes time(NULL), and any fixed seed gives the assignment that
+ * srand48(seed) would give the reference.  The output position is
+ * transform_point of the star (stVector.c:155-169): the star's direction at
+ * distance g = 5 log10(100 r) - 6.8, rotated so that the wedge's normal is +z.
+ * ------------------------------------------------------------------------ */
+
+typedef struct {
+    int number_streams;
+    int number_stars;
+    double epsilon_b;
+    double epsilon_s[MW_MAX_STREAMS];
+    double expected[MW_MAX_STREAMS];   /* nstars: sum over stars of p_s, in star order */
+    int separated[MW_MAX_STREAMS];     /* q: stars assigned to stream s */
+    double background_integral;        /* the integrals used */
+    double stream_integrals[MW_MAX_STREAMS];
+} mw_separation_summary;
+
+/* Separation of the context's stars for one parameter vector (params: the
+ * context's parameter count, or NULL for the parameter file's own values).
+ * integrals: [1 + S] background then stream integrals, or NULL to compute
+ * them first (one evaluation).  Outputs, each optional (NULL skips it):
+ * assignment [N], xyz [N][3], probability [N][S].  Needs 1 to 4 streams. */
+int mw_ctx_separation(mw_ctx* ctx, const double* params, int precision, const double* integrals,
+                      unsigned long seed, int* assignment, double* xyz, double* probability,
+                      mw_separation_summary* summary);
+
+/* Replaces separation (separation.c:37-254), which reads ap and sp from
+ * globals: writes "s x y z" per star to filename ("%d %lf %lf %lf", as
+ * :229) and prints the reference's summary lines to stdout.  Uses the
+ * parameter values held in ap and the drop-in precision. */
+int mw_hip_separation(const char* filename, double background_integral,
+                      const double* stream_integrals, const ASTRONOMY_PARAMETERS* ap,
+                      const STAR_POINTS* sp, unsigned long seed);
+
 /* Message of the last failure on this thread ("" if none). */
 const char* mw_last_error(void);
 

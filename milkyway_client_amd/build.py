@@ -22,12 +22,14 @@ LIB_NAME = "libmw_separation_hip.so"
 LIB_PATH = LIB_DIR / LIB_NAME
 ARCH = os.environ.get("MW_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["mw_kernels.hip", "mw_api.hip", "mw_batch.hip", "mw_server.hip"]
+HIP_SOURCES = ["mw_kernels.hip", "mw_api.hip", "mw_batch.hip", "mw_server.hip", "mw_separation.hip"]
 HOST_SOURCES = ["mw_tables.cpp"]
 C_SOURCES = ["mw_io.c"]
 CLIENT_SOURCE = "client/mw_client.c"
 CLIENT_NAME = "milkyway_separation_hip"
 CLIENT_PATH = LIB_DIR / CLIENT_NAME
+SEPARATE_SOURCE = "client/mw_separate.c"
+SEPARATE_NAME = "separation_hip"
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
@@ -78,12 +80,13 @@ def build(verbose: bool = False) -> Path:
     tmp = LIB_DIR / (LIB_NAME + ".tmp")
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", *objs, "-o", str(tmp)])
     os.replace(tmp, LIB_PATH)
-    # Command-line client, linked against the library with an $ORIGIN rpath
-    # so the pair can move together.
-    tmp = LIB_DIR / (CLIENT_NAME + ".tmp")
-    _run(["gcc", *GCC_FLAGS, *inc, str(CSRC / CLIENT_SOURCE), "-o", str(tmp), f"-L{LIB_DIR}",
-          "-lmw_separation_hip", "-Wl,-rpath,$ORIGIN"])
-    os.replace(tmp, CLIENT_PATH)
+    # Command-line programs, linked against the library with an $ORIGIN rpath
+    # so that they move together with it.
+    for src, name in ((CLIENT_SOURCE, CLIENT_NAME), (SEPARATE_SOURCE, SEPARATE_NAME)):
+        tmp = LIB_DIR / (name + ".tmp")
+        _run(["gcc", *GCC_FLAGS, *inc, str(CSRC / src), "-o", str(tmp), f"-L{LIB_DIR}",
+              "-lmw_separation_hip", "-Wl,-rpath,$ORIGIN"])
+        os.replace(tmp, LIB_DIR / name)
     # Byte-counter calibration probe (a plain copy kernel of known size).
     _run([hipcc, *HIPCC_FLAGS, str(CSRC / "hbm_copy_probe.hip"), "-o", str(LIB_DIR / "hbm_copy_probe")])
     return LIB_PATH

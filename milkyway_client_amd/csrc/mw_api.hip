@@ -317,7 +317,8 @@ std::mutex g_dropin_mutex;
 std::unique_ptr<mw_ctx> g_dropin;
 int g_precision = -1;
 
-int dropin_precision()
+// Caller holds g_dropin_mutex.
+int dropin_precision_locked()
 {
     if (g_precision < 0) {
         const char* e = std::getenv("MW_HIP_PRECISION");
@@ -360,6 +361,14 @@ int dropin_context(const ASTRONOMY_PARAMETERS* ap, const EVALUATION_STATE* es, c
 }
 
 }  // namespace
+
+namespace mw {
+int dropin_precision_value()
+{
+    std::lock_guard<std::mutex> lock(g_dropin_mutex);
+    return dropin_precision_locked();
+}
+}  // namespace mw
 
 // ===========================================================================
 // C ABI
@@ -526,7 +535,7 @@ int mw_hip_calculate_integrals(const ASTRONOMY_PARAMETERS* ap, EVALUATION_STATE*
         P.regions.push_back(PlanRegion{ 0, g, b, e });
     }
     P.integrals = true;
-    P.mixed = dropin_precision() == MW_PREC_MIXED;
+    P.mixed = dropin_precision_locked() == MW_PREC_MIXED;
     if ((rc = run_plan(c, P))) return rc;
     const int NI = integral_comps();
     for (size_t i = 0; i < P.regions.size(); i++) {
@@ -560,7 +569,7 @@ int mw_hip_calculate_likelihood(const ASTRONOMY_PARAMETERS* ap, EVALUATION_STATE
     P.K[0].I_bg = es->background_integral;
     for (int s = 0; s < ap->number_streams; s++) P.K[0].I_st[s] = es->stream_integrals[s];
     P.stars = true;
-    P.mixed = dropin_precision() == MW_PREC_MIXED;
+    P.mixed = dropin_precision_locked() == MW_PREC_MIXED;
     P.star_begin = std::max(0, es->current_star_point);
     if (P.star_begin > sp->number_stars) P.star_begin = sp->number_stars;
     if ((rc = run_plan(c, P))) return rc;

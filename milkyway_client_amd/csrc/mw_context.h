@@ -34,6 +34,11 @@ hipError_t launch_persistent(int bg, bool mixed, int variant, int grid, const Re
 hipError_t launch_reset_counters(unsigned* d_counters, int n, hipStream_t stream);
 hipError_t launch_star_any(int bg, bool mixed, const StarLaunch* d_items, int n_items, int n_blocks,
                            const EvalConst* d_consts, double* d_partials, hipStream_t stream);
+// Stream separation over every star of item (1 <= S <= 4): per star the
+// assignment, the transformed position [3] and the S membership probabilities.
+hipError_t launch_separation(int S, int bg, bool mixed, const StarLaunch& item, const EvalConst& k,
+                             const SepConst& sc, int* d_assign, double* d_xyz, double* d_sprob,
+                             hipStream_t stream);
 int persistent_variants();
 int tile_pairs();
 int integral_comps();
@@ -41,6 +46,10 @@ int star_comps();
 
 // Records msg as this thread's last error and returns code.
 int fail(int code, const std::string& msg);
+
+// Precision of the drop-in entry points (MW_HIP_PRECISION or
+// mw_hip_set_precision).
+int dropin_precision_value();
 
 #define MW_HIP_CHECK(expr)                                                                  \
     do {                                                                                    \
@@ -245,6 +254,8 @@ struct mw_ctx {
     mw::DevBuf<mw::ReduceItem> rred, sred;
     mw::DevBuf<double> ipart, spart, irows, srows, log10p;
     mw::DevBuf<int> first, count;
+    mw::DevBuf<int> sep_assign;         // stream separation outputs
+    mw::DevBuf<double> sep_xyz, sep_prob;
     mw::HostBuf<mw::EvalConst> h_consts;
     mw::HostBuf<mw::RegionLaunch> h_ritems;
     mw::HostBuf<mw::StarLaunch> h_sitems;

@@ -97,6 +97,15 @@ struct StarLaunch {
     const double* kw;          // [n_conv] W_i coeff exp(-dx_i^2 / (2 sigma^2))
 };
 
+// Constants of the stream separation pass (separation.c:37-254).
+struct SepConst {
+    double eps_b;                   // 1 / (1 + sum_s exp(w_s))
+    double eps_s[kMaxStreams];      // exp(w_s) / (1 + sum_s exp(w_s))
+    double I_bg, I_st[kMaxStreams]; // the evaluation's integrals
+    double rot[3][3];               // rotation taking the wedge's normal to +z
+    uint64_t x0;                    // drand48 state after srand48(seed)
+};
+
 // Second-pass item: the partials of blocks [block_begin, block_begin +
 // block_count) summed in block order into row out_index.
 struct ReduceItem {
@@ -113,6 +122,11 @@ void build_star_soa(const STAR_POINTS& sp, std::vector<double>* out);
 // Fills an EvalConst from the model parameters held in ap; returns false
 // (with a message) if the parameters cannot be evaluated here.
 bool make_eval_const(const ASTRONOMY_PARAMETERS& ap, EvalConst* k, std::string* err);
+
+// The stream separation's output frame: the wedge's normal (stripe_normal /
+// sgr_stripe_normal) and the rotation taking it to +z (get_transform).
+// Returns false for an unknown sgr_coordinates value.
+bool separation_frame(int wedge, int sgr, double normal[3], double rot[3][3]);
 
 // Star-pass constants shared by every star of a context: r_i / r_star and
 // W_i * coeff * exp(-dx_i^2 / (2 sigma^2)).

@@ -384,27 +384,25 @@ __global__ void reset_counters(unsigned* counters, int n)
 // ---------------------------------------------------------------------------
 // Star pass
 // ---------------------------------------------------------------------------
+
+// Convolved densities of star i (evaluation_optimized.c:190-299 for one star):
+// bg is the background density times reff(r) 3 sigma / r^3 (or -1 when
+// q == 0); st[s] is the stream density without that factor and without the
+// stream's on/off flag.
 template <int S, int BG, bool MIXED>
-__device__ __forceinline__ void star_block(const StarLaunch& it, long lb,
-                                           const EvalConst* __restrict__ consts,
-                                           const double* __restrict__ conv_ratio,
-                                           const double* __restrict__ conv_kw,
-                                           double* __restrict__ prow, double* __restrict__ log10p,
-                                           long log10p_stride, DD (*red)[kStarComps])
+__device__ __forceinline__ void star_densities(const double* __restrict__ star, long N, long i, int n,
+                                               const EvalConst& K,
+                                               const double* __restrict__ conv_ratio,
+                                               const double* __restrict__ conv_kw, double& reff,
+                                               double& bg, double (&st)[kMaxStreams])
 {
-    const EvalConst& K = consts[it.slot];
-    const long N = it.n_stars;
-    const long idx = it.star_begin + lb * kBlock + threadIdx.x;
-    const bool valid = idx < N;
-    const long i = valid ? idx : N - 1;
-    const int n = it.n_conv;
+    const Sight<S> sight = make_sight<S>(star[i], star[N + i], star[2 * N + i], K);
+    const double rstar = star[3 * N + i];
+    reff = star[4 * N + i];
 
-    const Sight<S> sight = make_sight<S>(it.star[i], it.star[N + i], it.star[2 * N + i], K);
-    const double rstar = it.star[3 * N + i];
-    const double reff = it.star[4 * N + i];
-
-    double bg = 0.0;
-    double st[kMaxStreams] = { 0.0, 0.0, 0.0, 0.0 };
+    bg = 0.0;
+#pragma unroll
+    for (int s = 0; s < kMaxStreams; s++) st[s] = 0.0;
     for (int j = 0; j < n; j++) {
         const double r = rstar * conv_ratio[j];
         const double qw = conv_kw[j] * (r * r * r);
@@ -424,6 +422,25 @@ __device__ __forceinline__ void star_block(const StarLaunch& it, long lb,
         }
         bg *= reff;
     }
+}
+
+template <int S, int BG, bool MIXED>
+__device__ __forceinline__ void star_block(const StarLaunch& it, long lb,
+                                           const EvalConst* __restrict__ consts,
+                                           const double* __restrict__ conv_ratio,
+                                           const double* __restrict__ conv_kw,
+                                           double* __restrict__ prow, double* __restrict__ log10p,
+                                           long log10p_stride, DD (*red)[kStarComps])
+{
+    const EvalConst& K = consts[it.slot];
+    const long N = it.n_stars;
+    const long idx = it.star_begin + lb * kBlock + threadIdx.x;
+    const bool valid = idx < N;
+    const long i = valid ? idx : N - 1;
+
+    double reff, bg;
+    double st[kMaxStreams];
+    star_densities<S, BG, MIXED>(it.star, N, i, it.n_conv, K, conv_ratio, conv_kw, reff, bg, st);
 
     // Mixture (evaluation_optimized.c:674-716).
     DD out[kStarComps];
@@ -498,6 +515,97 @@ __global__ __launch_bounds__(kBlock) void star_kernel_any(const StarLaunch* __re
     case 3: star_block<3, BG, MIXED>(it, lb, consts, it.ratio, it.kw, prow, nullptr, 0, red); break;
     default: star_block<4, BG, MIXED>(it, lb, consts, it.ratio, it.kw, prow, nullptr, 0, red); break;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Stream separation (separation.c:37-254): one work-item per star.
+// ---------------------------------------------------------------------------
+
+// drand48's generator (POSIX): X <- (a X + c) mod 2^48 with a = 0x5DEECE66D,
+// c = 0xB; each call returns X / 2^48 of the new state.  The k-th call after
+// srand48 is the affine map applied k times to the seeded state, built here
+// by binary powering of (a, c), so star i draws the same number as the i-th
+// sequential drand48() call in the reference's star loop.
+__device__ __forceinline__ double drand48_at(uint64_t x0, uint64_t k)
+{
+    constexpr uint64_t kMask = (1ull << 48) - 1;
+    uint64_t step_a = 0x5DEECE66Dull, step_c = 0xBull;
+    uint64_t acc_a = 1, acc_c = 0;
+    while (k) {
+        if (k & 1) {
+            acc_a = (acc_a * step_a) & kMask;
+            acc_c = (acc_c * step_a + step_c) & kMask;
+        }
+        step_c = (step_c * step_a + step_c) & kMask;
+        step_a = (step_a * step_a) & kMask;
+        k >>= 1;
+    }
+    const uint64_t x = (acc_a * x0 + acc_c) & kMask;
+    return (double)x * 0x1p-48;
+}
+
+// prob_ok (probability.c:310-403): 0 for the background, j + 1 for stream j,
+// from cumulative sums taken in the reference's order.  Where the reference
+// leaves its result unset (r == p[0] exactly with 2 or 3 streams, or NaN
+// probabilities), this returns what its 4-stream branch does: 1 and 0.
+template <int S>
+__device__ __forceinline__ int prob_ok(const double (&p)[kMaxStreams], double r)
+{
+    if constexpr (S == 1) {
+        return r > p[0] ? 0 : 1;
+    } else {
+        double cum[S];
+        cum[0] = p[0];
+#pragma unroll
+        for (int j = 1; j < S; j++) cum[j] = cum[j - 1] + p[j];
+        if (r > cum[S - 1]) return 0;
+        if (r <= p[0]) return 1;
+#pragma unroll
+        for (int j = 1; j < S; j++)
+            if (r > cum[j - 1] && r <= cum[j]) return j + 1;
+        return 0;
+    }
+}
+
+template <int S, int BG, bool MIXED>
+__global__ __launch_bounds__(kBlock) void separation_kernel(StarLaunch it, EvalConst K, SepConst C,
+                                                            int* __restrict__ assign,
+                                                            double* __restrict__ xyz,
+                                                            double* __restrict__ sprob)
+{
+    const long N = it.n_stars;
+    const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+
+    double reff, bg;
+    double st[kMaxStreams];
+    star_densities<S, BG, MIXED>(it.star, N, i, it.n_conv, K, it.ratio, it.kw, reff, bg, st);
+
+    // Membership probabilities (separation.c:170-189).
+    const double pbx = C.eps_b * bg / C.I_bg;
+    double psg[kMaxStreams], p[kMaxStreams];
+    double psg_sum = 0.0;
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        psg[s] = C.eps_s[s] * (st[s] * reff * K.on[s]) / C.I_st[s];
+        psg_sum += psg[s];
+    }
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        p[s] = psg[s] / (psg_sum + pbx);
+        sprob[i * S + s] = p[s];
+    }
+    assign[i] = prob_ok<S>(p, drand48_at(C.x0, (uint64_t)i + 1));
+
+    // Output position (separation.c:226-229): transform_point of the star's
+    // galactic xyz is rot (g u), with u the unit vector towards (l, b) and
+    // g = 5 log10(100 r) + 4.2 - 11 (stCoords.c:188-201, stVector.c:155-169:
+    // the point is rebuilt at distance g along the same direction, and the
+    // Sun's offset it carries cancels against the xsun subtraction).
+    const double g = 5.0 * (log(100.0 * it.star[3 * N + i]) / log(10.0)) + 4.2 - 11.0;
+    const double v[3] = { g * it.star[i], g * it.star[N + i], g * it.star[2 * N + i] };
+#pragma unroll
+    for (int k = 0; k < 3; k++) xyz[3 * i + k] = C.rot[k][0] * v[0] + C.rot[k][1] * v[1] + C.rot[k][2] * v[2];
 }
 
 // ---------------------------------------------------------------------------
@@ -602,6 +710,30 @@ static StarFn pick_star(int S, int bg, bool mixed)
     }
 }
 
+using SeparationFn = void (*)(StarLaunch, EvalConst, SepConst, int*, double*, double*);
+
+template <bool MIXED, int BG>
+SeparationFn separation_fn(int S)
+{
+    switch (S) {
+    case 1: return separation_kernel<1, BG, MIXED>;
+    case 2: return separation_kernel<2, BG, MIXED>;
+    case 3: return separation_kernel<3, BG, MIXED>;
+    default: return separation_kernel<4, BG, MIXED>;
+    }
+}
+
+static SeparationFn pick_separation(int S, int bg, bool mixed)
+{
+    if (mixed && bg == kBgHernquist) return separation_fn<true, kBgHernquist>(S);
+    switch (bg) {
+    case kBgHernquist: return separation_fn<false, kBgHernquist>(S);
+    case kBgHernquistAux: return separation_fn<false, kBgHernquistAux>(S);
+    case kBgGeneral: return separation_fn<false, kBgGeneral>(S);
+    default: return separation_fn<false, kBgZeroQ>(S);
+    }
+}
+
 hipError_t launch_integral(int S, int bg, bool mixed, const RegionLaunch* d_items, int n_items,
                            int n_blocks, const EvalConst* d_consts, double* d_partials,
                            hipStream_t stream)
@@ -620,6 +752,19 @@ hipError_t launch_star(int S, int bg, bool mixed, const StarLaunch* d_items, int
     if (n_blocks <= 0) return hipSuccess;
     hipLaunchKernelGGL(pick_star(S, bg, mixed), dim3(n_blocks), dim3(kBlock), 0, stream, d_items,
                        n_items, d_consts, d_ratio, d_kw, d_partials, d_log10p, stride);
+    return hipGetLastError();
+}
+
+hipError_t launch_separation(int S, int bg, bool mixed, const StarLaunch& item, const EvalConst& k,
+                             const SepConst& sc, int* d_assign, double* d_xyz, double* d_sprob,
+                             hipStream_t stream)
+{
+    if (S < 1 || S > kMaxStreams) return hipErrorInvalidValue;
+    if (item.n_stars <= 0) return hipSuccess;
+    const long n_blocks = (item.n_stars + kBlock - 1) / kBlock;
+    if (n_blocks > 0x7fffffffL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(pick_separation(S, bg, mixed), dim3((unsigned)n_blocks), dim3(kBlock), 0, stream,
+                       item, k, sc, d_assign, d_xyz, d_sprob);
     return hipGetLastError();
 }
 

@@ -10,6 +10,8 @@
 //   quadrature        milkyway_separation/cpu/numericalIntegration.c:67-104
 //   r-step tables     milkyway_separation/cpu/evaluation_optimized.c:159-188, :360-422
 //   constants         milkyway_separation/cpu/evaluation_optimized.c:54-133, :639-646
+//   separation frame  milkyway_separation/cpu/stCoords.c:142-185, stVector.c:109-140,
+//                     atSurveyGeometry.c:414-460
 // Compiled with -ffp-contract=off so the host arithmetic rounds like the
 // reference's scalar SSE2 code.
 #include <cmath>
@@ -273,6 +275,89 @@ void gc_to_lb(int wedge, int sgr, double mu, double nu, double* l, double* b)
         gc_to_sgr(mu, nu, wedge, &lam, &beta);
         sgr_to_gal(lam, beta, l, b);
     }
+}
+
+namespace {
+
+// Unit vector towards galactic (l, b) in degrees (lbToXyz, stCoords.c:177-185).
+void lb_unit(double l, double b, double* v)
+{
+    l = l / kDeg;
+    b = b / kDeg;
+    v[0] = std::cos(l) * std::cos(b);
+    v[1] = std::sin(l) * std::cos(b);
+    v[2] = std::sin(b);
+}
+
+// Survey (longitude, latitude) to equatorial, degrees (atSurveyToEq,
+// atSurveyGeometry.c:414-440): survey centre RA 185, Dec 32.5.
+void survey_to_equatorial(double slong, double slat, double* ra, double* dec)
+{
+    slong = slong * kPi / 180.0;
+    slat = slat * kPi / 180.0;
+    const double anode = (185.0 - 90.0) * kPi / 180.0;
+    const double eta_pole = 32.5 * kPi / 180.0;
+    const double x1 = -std::sin(slong);
+    const double y1 = std::cos(slat + eta_pole) * std::cos(slong);
+    const double z1 = std::sin(slat + eta_pole) * std::cos(slong);
+    *ra = std::atan2(y1, x1) + anode;
+    *dec = std::asin(z1);
+    *ra = *ra * 180.0 / kPi;
+    *dec = *dec * 180.0 / kPi;
+    wrap_lat_lon(dec, ra);
+}
+
+}  // namespace
+
+bool separation_frame(int wedge, int sgr, double normal[3], double rot[3][3])
+{
+    if (sgr == 0) {
+        // stripe_normal (stCoords.c:142-150): the pole of the stripe's great
+        // circle, at survey latitude 90 + eta.
+        const double eta = wedge * 2.5 - 57.5 - (wedge > 46 ? 180.0 : 0.0);
+        double ra, dec, l, b;
+        survey_to_equatorial(0.0, 90.0 + eta, &ra, &dec);
+        equatorial_to_galactic(ra, dec, &l, &b);
+        lb_unit(l, b, normal);
+    } else if (sgr == 1) {
+        // sgr_stripe_normal (stCoords.c:153-174): the cross product of the
+        // directions of (lambda, 0) and (lambda, 90), lambda = 2.5 wedge.
+        double l1, b1, l2, b2, a[3], c[3];
+        sgr_to_gal(wedge * 2.5, 0.0, &l1, &b1);
+        sgr_to_gal(wedge * 2.5, 90.0, &l2, &b2);
+        lb_unit(l1, b1, a);
+        lb_unit(l2, b2, c);
+        normal[0] = a[1] * c[2] - a[2] * c[1];
+        normal[1] = a[2] * c[0] - a[0] * c[2];
+        normal[2] = a[0] * c[1] - a[1] * c[0];
+    } else {
+        return false;
+    }
+    // get_transform(normal, +z) (stVector.c:109-140): the rotation about
+    // normal x z by the angle between them, through its quaternion.
+    const double t[3] = { 0.0, 0.0, 1.0 };
+    double axis[3] = { normal[1] * t[2] - normal[2] * t[1], normal[2] * t[0] - normal[0] * t[2],
+                       normal[0] * t[1] - normal[1] * t[0] };
+    const double an = std::sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2]);
+    axis[0] /= an;
+    axis[1] /= an;
+    axis[2] /= an;
+    const double fn = std::sqrt(normal[0] * normal[0] + normal[1] * normal[1] + normal[2] * normal[2]);
+    const double tn = std::sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+    const double dp = normal[0] * t[0] + normal[1] * t[1] + normal[2] * t[2];
+    const double angle = std::acos(dp / (fn * tn));
+    const double sa = std::sin(angle / 2);
+    const double x = axis[0] * sa, y = axis[1] * sa, z = axis[2] * sa, w = std::cos(angle / 2);
+    rot[0][0] = 1 - 2 * (y * y + z * z);
+    rot[0][1] = 2 * (x * y - z * w);
+    rot[0][2] = 2 * (x * z + y * w);
+    rot[1][0] = 2 * (x * y + z * w);
+    rot[1][1] = 1 - 2 * (x * x + z * z);
+    rot[1][2] = 2 * (y * z - x * w);
+    rot[2][0] = 2 * (x * z - y * w);
+    rot[2][1] = 2 * (y * z + x * w);
+    rot[2][2] = 1 - 2 * (x * x + y * y);
+    return true;
 }
 
 void lbr_to_xyz(const double* lbr, double* xyz)

@@ -46,6 +46,7 @@ EXPORTS = [
     "mw_free_star_points", "mw_initialize_state", "mw_free_state",
     "mw_get_optimized_parameter_count", "mw_set_astronomy_parameters",
     "mw_write_astronomy_parameters", "mw_write_star_points",
+    "mw_ctx_separation", "mw_hip_separation",
 ]
 
 
@@ -73,6 +74,14 @@ class mw_timing(C.Structure):
 class mw_server_config(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_size_t), ("max_inflight", C.c_int),
                 ("max_batch_vectors", C.c_int), ("variant", C.c_int), ("blocks_per_cu", C.c_int)]
+
+
+class mw_separation_summary(C.Structure):
+    _fields_ = [("number_streams", C.c_int), ("number_stars", C.c_int),
+                ("epsilon_b", C.c_double), ("epsilon_s", C.c_double * MAX_STREAMS),
+                ("expected", C.c_double * MAX_STREAMS), ("separated", C.c_int * MAX_STREAMS),
+                ("background_integral", C.c_double),
+                ("stream_integrals", C.c_double * MAX_STREAMS)]
 
 
 class mw_request_times(C.Structure):
@@ -180,6 +189,11 @@ def load_library() -> C.CDLL:
     L.mw_write_astronomy_parameters.restype = C.c_int
     L.mw_write_star_points.argtypes = [C.c_char_p, SP]
     L.mw_write_star_points.restype = C.c_int
+    L.mw_ctx_separation.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_ulong, C.POINTER(C.c_int),
+                                    _dp, _dp, C.POINTER(mw_separation_summary)]
+    L.mw_ctx_separation.restype = C.c_int
+    L.mw_hip_separation.argtypes = [C.c_char_p, C.c_double, _dp, AP, SP, C.c_ulong]
+    L.mw_hip_separation.restype = C.c_int
     _lib = L
     return L
 
@@ -220,6 +234,14 @@ def calculate_integrals(params: ParamsHolder, state: StateHolder, stars: Optiona
 
 def calculate_likelihood(params: ParamsHolder, state: StateHolder, stars: StarsHolder):
     _check(load_library().mw_hip_calculate_likelihood(params.ptr, state.ptr, stars.ptr))
+
+
+def hip_separation(filename, background_integral: float, stream_integrals,
+                   params: ParamsHolder, stars: StarsHolder, seed: int) -> None:
+    """separation() replacement (mw_hip_separation): writes the output file."""
+    si = np.ascontiguousarray(stream_integrals, dtype=np.float64)
+    _check(load_library().mw_hip_separation(str(filename).encode(), background_integral,
+                                            si.ctypes.data_as(_dp), params.ptr, stars.ptr, seed))
 
 
 class Result:
@@ -298,6 +320,32 @@ class Context:
         t = mw_timing()
         _check(load_library().mw_ctx_last_timing(self.handle, C.byref(t)))
         return t
+
+    def separation(self, vector=None, precision: int = PREC_FP64, integrals=None, seed: int = 0):
+        """Stream separation (mw_ctx_separation): per-star assignment [N],
+        position [N][3] and membership probabilities [N][S], and the summary."""
+        S = self._params.ap.number_streams
+        N = self.n_stars
+        vec = None
+        if vector is not None:
+            vec = np.ascontiguousarray(vector, dtype=np.float64)
+            if vec.size != self.n_params:
+                raise ValueError(f"parameter vector needs {self.n_params} values, got {vec.size}")
+        ints = None
+        if integrals is not None:
+            ints = np.ascontiguousarray(integrals, dtype=np.float64)
+            if ints.size != 1 + S:
+                raise ValueError(f"integrals need {1 + S} values")
+        assign = np.zeros(max(1, N), dtype=np.int32)
+        xyz = np.zeros((max(1, N), 3))
+        prob = np.zeros((max(1, N), max(1, S)))
+        summ = mw_separation_summary()
+        _check(load_library().mw_ctx_separation(
+            self.handle, vec.ctypes.data_as(_dp) if vec is not None else None, precision,
+            ints.ctypes.data_as(_dp) if ints is not None else None, seed,
+            assign.ctypes.data_as(C.POINTER(C.c_int)), xyz.ctypes.data_as(_dp),
+            prob.ctypes.data_as(_dp), C.byref(summ)))
+        return assign[:N], xyz[:N], prob[:N, :S], summ
 
 
 BATCH_GRAPH = 1

@@ -998,3 +998,238 @@ int orc_calculate_likelihood_omp(const ASTRONOMY_PARAMETERS* ap, EVALUATION_STAT
     orc_free_constants(&k0);
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Stream separation: separation.c:37-254 with probability.c:310-403,       */
+/* stCoords.c:52-65, :142-201, stVector.c:60-169, atSurveyGeometry.c:414-460 */
+/* ------------------------------------------------------------------------ */
+
+/* atSurveyGeometry.c:444-460 */
+static double orc_eta_from_stripe(int wedge)
+{
+    double stripe_separation = 2.5;
+    if (wedge <= 46) return wedge * stripe_separation - 57.5;
+    return wedge * stripe_separation - 57.5 - 180.0;
+}
+
+/* atSurveyGeometry.c:414-440; at_deg2Rad / at_rad2Deg expand to
+ * x * DPI / 180.0 and x * 180.0 / DPI. */
+static void orc_survey_to_eq(double slong, double slat, double* ra, double* dec)
+{
+    double anode, eta_pole, x1, y1, z1;
+    double center_ra = 185.0, center_dec = 32.5;
+    slong = slong * ORC_PI / 180.0;
+    slat = slat * ORC_PI / 180.0;
+    anode = center_ra - 90.0;
+    anode = anode * ORC_PI / 180.0;
+    eta_pole = center_dec * ORC_PI / 180.0;
+    x1 = -sin(slong);
+    y1 = cos(slat + eta_pole) * cos(slong);
+    z1 = sin(slat + eta_pole) * cos(slong);
+    *ra = atan2(y1, x1) + anode;
+    *dec = asin(z1);
+    *ra = *ra * 180.0 / ORC_PI;
+    *dec = *dec * 180.0 / ORC_PI;
+    orc_bound2(dec, ra);
+}
+
+/* stCoords.c:177-185 */
+static void orc_lb_to_xyz(double l, double b, double* xyz)
+{
+    l = l / ORC_DEG;
+    b = b / ORC_DEG;
+    xyz[0] = cos(l) * cos(b);
+    xyz[1] = sin(l) * cos(b);
+    xyz[2] = sin(b);
+}
+
+/* stCoords.c:142-150 */
+void orc_stripe_normal(int wedge, double* xyz)
+{
+    double eta, ra, dec, l, b;
+    eta = orc_eta_from_stripe(wedge);
+    orc_survey_to_eq(0, 90.0 + eta, &ra, &dec);
+    orc_eq_to_gal(ra, dec, &l, &b);
+    orc_lb_to_xyz(l, b, xyz);
+}
+
+/* stCoords.c:153-174 */
+void orc_sgr_stripe_normal(int wedge, double* xyz)
+{
+    double l1, b1, l2, b2, xyz1[3], xyz2[3];
+    orc_sgr_to_gal(wedge * 2.5, 0, &l1, &b1);
+    orc_sgr_to_gal(wedge * 2.5, 90, &l2, &b2);
+    orc_lb_to_xyz(l1, b1, xyz1);
+    orc_lb_to_xyz(l2, b2, xyz2);
+    xyz[0] = xyz1[1] * xyz2[2] - xyz1[2] * xyz2[1];
+    xyz[1] = xyz1[2] * xyz2[0] - xyz1[0] * xyz2[2];
+    xyz[2] = xyz1[0] * xyz2[1] - xyz1[1] * xyz2[0];
+}
+
+/* stVector.c:60-105: norm, dotp, crossp */
+static double orc_norm(const double* v) { return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+static double orc_dotp(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+/* stVector.c:109-140; mat is row-major [3][3] */
+void orc_get_transform(const double* f, const double* t, double* mat)
+{
+    double axis[3], angle, sin_a, x, y, z, w, n;
+    axis[0] = f[1] * t[2] - f[2] * t[1];
+    axis[1] = f[2] * t[0] - f[0] * t[2];
+    axis[2] = f[0] * t[1] - f[1] * t[0];
+    n = orc_norm(axis);
+    axis[0] /= n;
+    axis[1] /= n;
+    axis[2] /= n;
+    angle = acos(orc_dotp(f, t) / (orc_norm(f) * orc_norm(t)));
+    sin_a = sin(angle / 2);
+    x = axis[0] * sin_a;
+    y = axis[1] * sin_a;
+    z = axis[2] * sin_a;
+    w = cos(angle / 2);
+    mat[0] = 1 - 2 * (y * y + z * z);
+    mat[1] = 2 * (x * y - z * w);
+    mat[2] = 2 * (x * z + y * w);
+    mat[3] = 2 * (x * y + z * w);
+    mat[4] = 1 - 2 * (x * x + z * z);
+    mat[5] = 2 * (y * z - x * w);
+    mat[6] = 2 * (x * z - y * w);
+    mat[7] = 2 * (y * z + x * w);
+    mat[8] = 1 - 2 * (x * x + y * y);
+}
+
+/* stCoords.c:52-65 */
+static void orc_xyz2lbr(const double* xyz, double* lbr)
+{
+    double xsun = xyz[0] + orc_sun_r0;
+    double temp = xsun * xsun + xyz[1] * xyz[1];
+    lbr[0] = atan2(xyz[1], xsun) * ORC_DEG;
+    lbr[1] = atan2(xyz[2], sqrt(temp)) * ORC_DEG;
+    lbr[2] = sqrt(temp + xyz[2] * xyz[2]);
+    if (lbr[0] < 0) lbr[0] += 360;
+}
+
+/* stVector.c:155-169 with xyz_mag / xyz2lbg (stCoords.c:188-201) and
+ * do_transform (stVector.c:143-153) */
+void orc_transform_point(const double* point, const double* cmat, const double* xsun, double* out)
+{
+    double lbg[3], p[3];
+    orc_xyz2lbr(point, lbg);
+    lbg[2] = 5.0 * (log(100.0 * lbg[2]) / log(10.0)) + 4.2 - 11.0;
+    orc_lbr2xyz(lbg, out);
+    p[0] = out[0] - xsun[0];
+    p[1] = out[1] - xsun[1];
+    p[2] = out[2] - xsun[2];
+    out[0] = orc_dotp(cmat, p);
+    out[1] = orc_dotp(cmat + 3, p);
+    out[2] = orc_dotp(cmat + 6, p);
+}
+
+/* probability.c:310-396 with the random number passed in.  Returns -1 where
+ * the reference leaves its result unset (n = 2 or 3 with r == p[0] or NaN
+ * probabilities, n = 4 with NaN), and -2 for n outside 1-4 (the reference
+ * prints an error and exits). */
+int orc_prob_ok(int n, const double* p, double r)
+{
+    double step1, step2, step3;
+    int ok = -1;
+    switch (n) {
+    case 1:
+        ok = (r > p[0]) ? 0 : 1;
+        break;
+    case 2:
+        step1 = p[0] + p[1];
+        if (r > step1) ok = 0;
+        else if (r < p[0]) ok = 1;
+        else if ((r > p[0]) && (r <= step1)) ok = 2;
+        break;
+    case 3:
+        step1 = p[0] + p[1];
+        step2 = p[0] + p[1] + p[2];
+        if (r > step2) ok = 0;
+        else if (r < p[0]) ok = 1;
+        else if ((r > p[0]) && (r <= step1)) ok = 2;
+        else if ((r > step1) && (r <= step2)) ok = 3;
+        break;
+    case 4:
+        step1 = p[0] + p[1];
+        step2 = p[0] + p[1] + p[2];
+        step3 = p[0] + p[1] + p[2] + p[3];
+        if (r > step3) ok = 0;
+        else if (r <= p[0]) ok = 1;
+        else if ((r > p[0]) && (r <= step1)) ok = 2;
+        else if ((r > step1) && (r <= step2)) ok = 3;
+        else if ((r > step2) && (r <= step3)) ok = 4;
+        break;
+    default:
+        ok = -2;
+    }
+    return ok;
+}
+
+/* separation.c:37-254 (twoPanel == 1).  The star loop's probabilities come
+ * from the same set_probability_constants / calculate_probabilities pair as
+ * the likelihood pass (the call at separation.c:162-164 predates their
+ * current signatures).  The random numbers are glibc's drand48 sequence after
+ * srand48(seed) (erand48 on a state seeded the same way).  Outputs:
+ * assign[N], xyz[N][3], sprob[N][S], nstars[S], q[S]; sprob may be NULL. */
+int orc_separation(const ASTRONOMY_PARAMETERS* ap, const STAR_POINTS* sp, double background_integral,
+                   const double* stream_integrals, unsigned long seed, int* assign, double* xyz,
+                   double* sprob, double* nstars, int* q)
+{
+    OrcConst k;
+    int S = ap->number_streams, i, j;
+    double dnormal[3], dortho[3] = { 0.0, 0.0, 1.0 }, xsun[3] = { -8.5, 0.0, 0.0 }, cmatrix[9];
+    double epsilon_s[ORC_MAX_STREAMS], epsilon_b, denom, prob_b;
+    double prob_s[ORC_MAX_STREAMS], psg[ORC_MAX_STREAMS], p[ORC_MAX_STREAMS];
+    unsigned short xsubi[3];
+    if (S < 1 || S > 4) return 1;
+    if (ap->sgr_coordinates == 0)
+        orc_stripe_normal(ap->wedge, dnormal);
+    else if (ap->sgr_coordinates == 1)
+        orc_sgr_stripe_normal(ap->wedge, dnormal);
+    else
+        return 1;
+    orc_get_transform(dnormal, dortho, cmatrix);
+
+    denom = 1.0;
+    for (j = 0; j < S; j++) denom += exp(ap->stream_weights[j]);
+    for (j = 0; j < S; j++) epsilon_s[j] = exp(ap->stream_weights[j]) / denom;
+    epsilon_b = 1.0 / denom;
+
+    if (orc_init_constants(ap, &k)) return 1;
+    int n = k.convolve;
+    double* t = (double*)malloc(sizeof(double) * 4 * n);
+    double reff;
+    memset(k.xyz, 0, sizeof(double) * 3 * n);
+
+    /* srand48(seed): x = (seed & 0xffffffff) << 16 | 0x330e */
+    xsubi[0] = 0x330e;
+    xsubi[1] = (unsigned short)(seed & 0xffff);
+    xsubi[2] = (unsigned short)((seed >> 16) & 0xffff);
+
+    for (j = 0; j < S; j++) {
+        nstars[j] = 0;
+        q[j] = 0;
+    }
+    for (i = 0; i < sp->number_stars; i++) {
+        const double* star = sp->stars[i];
+        double starxyz[3], psg_sum = 0;
+        orc_prob_constants(&k, star[2], t, t + n, t + 2 * n, t + 3 * n, &reff);
+        orc_probabilities(&k, t, t + n, t + 2 * n, t + 3 * n, reff, star, &prob_b, prob_s);
+        double pbx = epsilon_b * prob_b / background_integral;
+        for (j = 0; j < S; j++) psg[j] = epsilon_s[j] * prob_s[j] / stream_integrals[j];
+        for (j = 0; j < S; j++) psg_sum += psg[j];
+        for (j = 0; j < S; j++) p[j] = psg[j] / (psg_sum + pbx);
+        for (j = 0; j < S; j++) nstars[j] += p[j];
+        if (sprob)
+            for (j = 0; j < S; j++) sprob[(size_t)i * S + j] = p[j];
+        assign[i] = orc_prob_ok(S, p, erand48(xsubi));
+        if (assign[i] >= 1) q[assign[i] - 1]++;
+        orc_lbr2xyz(star, starxyz);
+        orc_transform_point(starxyz, cmatrix, xsun, xyz + 3 * (size_t)i);
+    }
+    free(t);
+    orc_free_constants(&k);
+    return 0;
+}

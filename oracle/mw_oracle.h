@@ -89,6 +89,17 @@ int orc_calculate_integrals_omp(const ASTRONOMY_PARAMETERS* ap, EVALUATION_STATE
 int orc_calculate_likelihood_omp(const ASTRONOMY_PARAMETERS* ap, EVALUATION_STATE* es,
                                  const STAR_POINTS* sp, int n_threads);
 
+/* Stream separation (separation.c:37-254) and its geometry helpers.
+ * mat / cmat are row-major [3][3]. */
+void orc_stripe_normal(int wedge, double* xyz);
+void orc_sgr_stripe_normal(int wedge, double* xyz);
+void orc_get_transform(const double* f, const double* t, double* mat);
+void orc_transform_point(const double* point, const double* cmat, const double* xsun, double* out);
+int orc_prob_ok(int n, const double* p, double r);
+int orc_separation(const ASTRONOMY_PARAMETERS* ap, const STAR_POINTS* sp, double background_integral,
+                   const double* stream_integrals, unsigned long seed, int* assign, double* xyz,
+                   double* sprob, double* nstars, int* q);
+
 #ifdef __cplusplus
 }
 #endif

@@ -70,6 +70,15 @@ def lib() -> C.CDLL:
         L.orc_calculate_integrals_omp.restype = C.c_int
         L.orc_calculate_likelihood_omp.argtypes = [AP, ES, SP, C.c_int]
         L.orc_calculate_likelihood_omp.restype = C.c_int
+        L.orc_stripe_normal.argtypes = [C.c_int, _dp]
+        L.orc_sgr_stripe_normal.argtypes = [C.c_int, _dp]
+        L.orc_get_transform.argtypes = [_dp, _dp, _dp]
+        L.orc_transform_point.argtypes = [_dp, _dp, _dp, _dp]
+        L.orc_prob_ok.argtypes = [C.c_int, _dp, C.c_double]
+        L.orc_prob_ok.restype = C.c_int
+        L.orc_separation.argtypes = [AP, SP, C.c_double, _dp, C.c_ulong, C.POINTER(C.c_int), _dp,
+                                     _dp, _dp, C.POINTER(C.c_int)]
+        L.orc_separation.restype = C.c_int
         _lib = L
     return _lib
 
@@ -225,3 +234,66 @@ def wedge_incl(wedge: int) -> float:
 
 def get_node() -> float:
     return lib().orc_get_node()
+
+
+def stripe_normal(wedge: int, sgr: int = 0) -> np.ndarray:
+    """stCoords.c:142-174: the wedge's normal (SDSS or Sagittarius stripe)."""
+    v = np.zeros(3)
+    (lib().orc_sgr_stripe_normal if sgr else lib().orc_stripe_normal)(wedge, _ptr(v))
+    return v
+
+
+def get_transform(f, t) -> np.ndarray:
+    """stVector.c:109-140: rotation matrix taking f to t."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    m = np.zeros(9)
+    lib().orc_get_transform(_ptr(f), _ptr(t), _ptr(m))
+    return m.reshape(3, 3)
+
+
+def transform_point(point, cmat, xsun=(-8.5, 0.0, 0.0)) -> np.ndarray:
+    """stVector.c:155-169 on a galactocentric point."""
+    p = np.ascontiguousarray(point, dtype=np.float64)
+    m = np.ascontiguousarray(cmat, dtype=np.float64).reshape(9)
+    xs = np.ascontiguousarray(xsun, dtype=np.float64)
+    out = np.zeros(3)
+    lib().orc_transform_point(_ptr(p), _ptr(m), _ptr(xs), _ptr(out))
+    return out
+
+
+def prob_ok(p, r: float) -> int:
+    """probability.c:310-396 with the random number r given (-1 where the
+    reference leaves the result unset)."""
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    return lib().orc_prob_ok(len(p), _ptr(p), r)
+
+
+class SeparationResult:
+    def __init__(self, assign, xyz, sprob, nstars, q):
+        self.assignment = assign
+        self.xyz = xyz
+        self.probability = sprob
+        self.expected = nstars
+        self.separated = q
+
+
+def separation(params: ParamsHolder, stars: StarsHolder, background_integral: float,
+               stream_integrals, seed: int) -> SeparationResult:
+    """separation.c:37-254 for the stars with the given integrals and
+    srand48(seed)."""
+    S = params.ap.number_streams
+    N = len(stars)
+    si = np.ascontiguousarray(stream_integrals, dtype=np.float64)
+    assign = np.zeros(max(1, N), dtype=np.int32)
+    xyz = np.zeros((max(1, N), 3))
+    sprob = np.zeros((max(1, N), max(1, S)))
+    nstars = np.zeros(max(1, S))
+    q = np.zeros(max(1, S), dtype=np.int32)
+    ip = C.POINTER(C.c_int)
+    rc = lib().orc_separation(params.ptr, stars.ptr, background_integral, _ptr(si), seed,
+                              assign.ctypes.data_as(ip), _ptr(xyz), _ptr(sprob), _ptr(nstars),
+                              q.ctypes.data_as(ip))
+    if rc:
+        raise ValueError("oracle rejected the separation inputs")
+    return SeparationResult(assign[:N], xyz[:N], sprob[:N, :S], nstars[:S], q[:S])

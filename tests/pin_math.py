@@ -221,3 +221,66 @@ def evaluate(p, stars):
     bg, st, regs = integrals(p)
     lp = star_log10p(p, stars, bg, st)
     return dict(I_bg=bg, I_st=st, regions=regs, log10p=lp, likelihood=float(np.sum(lp)) / len(stars))
+
+
+# ---------------------------------------------------------------------------
+# Stream separation, from the mathematics
+# ---------------------------------------------------------------------------
+
+def sgr_lb(lam_deg, beta_deg):
+    """Galactic (l, b) of Sagittarius (lambda, beta) through the rotation
+    matrix, with the reference's axis convention (z = -sin beta)."""
+    lam, beta = lam_deg * D2R, beta_deg * D2R
+    v = np.array([np.cos(beta) * np.cos(lam), np.cos(beta) * np.sin(lam), -np.sin(beta)])
+    g = sgr_rotation().T @ v
+    return np.arctan2(g[1], g[0]) / D2R, np.arcsin(np.clip(g[2], -1, 1)) / D2R
+
+
+def stripe_normal(wedge: int, sgr: int) -> np.ndarray:
+    """SDSS: the pole (nu = 90) of the stripe's great circle.  Sagittarius:
+    the normal of the plane through the directions of (lambda, 0) and
+    (lambda, 90), lambda = 2.5 wedge, oriented as their cross product."""
+    if sgr:
+        a = unit_los(*sgr_lb(2.5 * wedge, 0.0))
+        c = unit_los(*sgr_lb(2.5 * wedge, 90.0))
+        n = np.cross(a, c)
+        return n / np.linalg.norm(n)
+    l, b = sdss_gc_to_lb(wedge, np.array([0.0]), np.array([90.0]))
+    return unit_los(l[0], b[0])
+
+
+def rotation_to_z(n) -> np.ndarray:
+    """Rodrigues' formula: the rotation about n x z by the angle between n and
+    z, R = I + sin(t) K + (1 - cos(t)) K^2 with K the cross-product matrix of
+    the unit axis."""
+    n = np.asarray(n, dtype=np.float64) / np.linalg.norm(n)
+    z = np.array([0.0, 0.0, 1.0])
+    axis = np.cross(n, z)
+    s = np.linalg.norm(axis)
+    k = axis / s
+    t = np.arctan2(s, n @ z)
+    K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+    return np.eye(3) + np.sin(t) * K + (1 - np.cos(t)) * (K @ K)
+
+
+def separation_positions(stars, rot) -> np.ndarray:
+    """A star at (l, b, r) is drawn at distance g = 5 log10(100 r) + 4.2 - 11
+    (the magnitude-like log distance) along its direction, in the rotated frame."""
+    stars = np.asarray(stars, dtype=np.float64)
+    g = 5.0 * np.log10(100.0 * stars[:, 2]) + 4.2 - 11.0
+    return (g[:, None] * unit_los(stars[:, 0], stars[:, 1])) @ rot.T
+
+
+def membership(p, stars, I_bg, I_st, model: Model = None) -> np.ndarray:
+    """Posterior stream-membership probabilities of each star, [N][S]:
+    w_s S_s / I_s over (sum_t w_t S_t / I_t + B / I_b), with w_s = exp(weight_s)
+    and the background's weight fixed at 1 (the common normalisation of the
+    mixture weights cancels)."""
+    model = model or Model(p)
+    stars = np.asarray(stars, dtype=np.float64)
+    rho, ri, qw, gi = distance_tables(stars[:, 2], p.convolve)
+    u = unit_los(stars[:, 0], stars[:, 1])
+    bg, st = model.densities(u, ri, qw, gi)
+    terms = np.stack([model.ws[s] * st[s] * rho / I_st[s] for s in range(len(p.streams))], axis=1)
+    bgp = -np.ones(len(stars)) if bg is None else bg * rho
+    return terms / (terms.sum(axis=1) + bgp / I_bg)[:, None]
