@@ -51,6 +51,25 @@ int mw_initialize_state(const ASTRONOMY_PARAMETERS* ap, const STAR_POINTS* sp,
 /* Replaces free_state (evaluation_state.c). */
 void mw_free_state(EVALUATION_STATE* es);
 
+/* Replaces write_checkpoint (milkyway_separation/cpu/evaluation_state.c:214-257,
+ * areas :33-66): the state's totals, star-pass position and every area's
+ * geometry, calculation range and partial integrals, to filename (the
+ * reference resolves "astronomy_checkpoint" through BOINC).  Returns 0, or 1
+ * with a message on stderr. */
+int mw_write_checkpoint(const char* filename, const EVALUATION_STATE* es);
+
+/* Replaces read_checkpoint (evaluation_state.c:259-290, areas :68-95) into a
+ * state set up by mw_initialize_state for the same model.  A missing file is
+ * not an error (returns 0 and leaves es unchanged), as in the reference; a
+ * file that does not parse or does not match es's stream and region counts
+ * returns 1. */
+int mw_read_checkpoint(const char* filename, EVALUATION_STATE* es);
+
+/* Replaces calculate_progress (evaluation_optimized.c:301-333): the fraction
+ * of integral cells and stars done, from current_integral, each area's
+ * current_calculation and current_star_point. */
+double mw_calculate_progress(const EVALUATION_STATE* es);
+
 /* Replaces get_optimized_parameter_count (parameters.c:307-330). */
 int mw_get_optimized_parameter_count(const ASTRONOMY_PARAMETERS* ap);
 

@@ -22,14 +22,17 @@ LIB_NAME = "libmw_separation_hip.so"
 LIB_PATH = LIB_DIR / LIB_NAME
 ARCH = os.environ.get("MW_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["mw_kernels.hip", "mw_api.hip", "mw_batch.hip", "mw_server.hip", "mw_separation.hip"]
+HIP_SOURCES = ["mw_kernels.hip", "mw_api.hip", "mw_batch.hip", "mw_server.hip", "mw_separation.hip",
+               "mw_search.hip"]
 HOST_SOURCES = ["mw_tables.cpp"]
-C_SOURCES = ["mw_io.c"]
+C_SOURCES = ["mw_io.c", "mw_search_host.c"]
 CLIENT_SOURCE = "client/mw_client.c"
 CLIENT_NAME = "milkyway_separation_hip"
 CLIENT_PATH = LIB_DIR / CLIENT_NAME
 SEPARATE_SOURCE = "client/mw_separate.c"
 SEPARATE_NAME = "separation_hip"
+ERRORS_SOURCE = "client/mw_errors.c"
+ERRORS_NAME = "errors_hip"
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
@@ -82,7 +85,8 @@ def build(verbose: bool = False) -> Path:
     os.replace(tmp, LIB_PATH)
     # Command-line programs, linked against the library with an $ORIGIN rpath
     # so that they move together with it.
-    for src, name in ((CLIENT_SOURCE, CLIENT_NAME), (SEPARATE_SOURCE, SEPARATE_NAME)):
+    for src, name in ((CLIENT_SOURCE, CLIENT_NAME), (SEPARATE_SOURCE, SEPARATE_NAME),
+                      (ERRORS_SOURCE, ERRORS_NAME)):
         tmp = LIB_DIR / (name + ".tmp")
         _run(["gcc", *GCC_FLAGS, *inc, str(CSRC / src), "-o", str(tmp), f"-L{LIB_DIR}",
               "-lmw_separation_hip", "-Wl,-rpath,$ORIGIN"])

@@ -584,20 +584,9 @@ int mw_hip_calculate_likelihood(const ASTRONOMY_PARAMETERS* ap, EVALUATION_STATE
 
 double mw_hip_evaluate(double* parameters, ASTRONOMY_PARAMETERS* ap, EVALUATION_STATE* es, STAR_POINTS* sp)
 {
-    // set_astronomy_parameters + reset_evaluation_state (evaluation_state.c:178-199)
+    // set_astronomy_parameters + reset_evaluation_state
     apply_vector(*ap, parameters);
-    es->current_integral = 0;
-    es->background_integral = 0;
-    for (int s = 0; s < es->number_streams; s++) es->stream_integrals[s] = 0;
-    es->current_star_point = 0;
-    es->num_zero = 0;
-    es->bad_jacobians = 0;
-    es->prob_sum = 0;
-    for (int g = 0; g < es->number_integrals; g++) {
-        es->integral[g]->background_integral = 0;
-        for (int s = 0; s < es->integral[g]->number_streams; s++) es->integral[g]->stream_integrals[s] = 0;
-        es->integral[g]->current_calculation = es->integral[g]->min_calculation;
-    }
+    reset_state(es);
     int rc = mw_hip_calculate_integrals(ap, es, sp);
     if (rc) {
         fprintf(stderr, "APP: error calculating integrals: %d (%s)\n", rc, mw_last_error());

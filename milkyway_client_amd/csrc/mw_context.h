@@ -223,6 +223,25 @@ inline void apply_vector(ASTRONOMY_PARAMETERS& ap, const double* v)
     }
 }
 
+// reset_evaluation_state (evaluation_state.c:170-190); every stream
+// integral is cleared (the reference's memset covers number_streams bytes,
+// and the integral pass overwrites the totals anyway).
+inline void reset_state(EVALUATION_STATE* es)
+{
+    es->current_integral = 0;
+    es->background_integral = 0;
+    for (int s = 0; s < es->number_streams; s++) es->stream_integrals[s] = 0;
+    es->current_star_point = 0;
+    es->num_zero = 0;
+    es->bad_jacobians = 0;
+    es->prob_sum = 0;
+    for (int g = 0; g < es->number_integrals; g++) {
+        es->integral[g]->background_integral = 0;
+        for (int s = 0; s < es->integral[g]->number_streams; s++) es->integral[g]->stream_integrals[s] = 0;
+        es->integral[g]->current_calculation = es->integral[g]->min_calculation;
+    }
+}
+
 struct RegionDev {
     RegionGeom geom;
     int n_conv = 0;

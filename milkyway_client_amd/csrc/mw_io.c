@@ -466,6 +466,162 @@ void mw_free_state(EVALUATION_STATE* es)
     memset(es, 0, sizeof *es);
 }
 
+/* fwrite_integral_area (evaluation_state.c:33-66): geometry with three
+ * decimals, the range, the partial integrals with twenty. */
+static void write_area(FILE* f, const INTEGRAL_AREA* a)
+{
+    fprintf(f, "mu[min,max,steps]: %.3f, %.3f, %d\n", a->mu_min, a->mu_max, a->mu_steps);
+    fprintf(f, "nu[min,max,steps]: %.3f, %.3f, %d\n", a->nu_min, a->nu_max, a->nu_steps);
+    fprintf(f, " r[min,max,steps]: %.3f, %.3f, %d\n", a->r_min, a->r_max, a->r_steps);
+    fprintf(f, "min_calculation: %ld, max_calculation: %ld, current_calculation: %ld\n", a->min_calculation,
+            a->max_calculation, a->current_calculation);
+    fprintf(f, "background_integral: %.20f\n", a->background_integral);
+    write_double_array(f, "stream_integrals", a->number_streams, a->stream_integrals);
+}
+
+int mw_write_checkpoint(const char* filename, const EVALUATION_STATE* es)
+{
+    FILE* f;
+    int i;
+    if (!filename || !es) return 1;
+    f = fopen(filename, "w+");
+    if (!f) {
+        fprintf(stderr, "APP: error writing checkpoint (opening checkpoint file)\n");
+        return 1;
+    }
+    fprintf(f, "background_integral: %.20f\n", es->background_integral);
+    fprintf(f, "stream_integrals[%d]: ", es->number_streams);
+    for (i = 0; i < es->number_streams; i++)
+        fprintf(f, i + 1 < es->number_streams ? "%.20f, " : "%.20f", es->stream_integrals[i]);
+    fprintf(f, "\n");
+    fprintf(f, "prob_sum: %.20f, num_zero: %d, bad_jacobians: %d\n", es->prob_sum, es->num_zero,
+            es->bad_jacobians);
+    fprintf(f, "current_star_point: %d\n", es->current_star_point);
+    fprintf(f, "current_integral: %d\n", es->current_integral);
+    fprintf(f, "number_integrals: %d\n", es->number_integrals);
+    for (i = 0; i < es->number_integrals; i++) write_area(f, es->integral[i]);
+    if (fclose(f)) {
+        fprintf(stderr, "APP: error writing checkpoint (closing checkpoint file)\n");
+        return 1;
+    }
+    return 0;
+}
+
+/* n values after "name[n]:" into out, which holds exactly n. */
+static int read_fixed_array(FILE* f, const char* name, int n, double* out)
+{
+    char fmt[96];
+    int got = -1, i;
+    snprintf(fmt, sizeof fmt, " %s [%%d] :", name);
+    if (fscanf(f, fmt, &got) != 1 || got != n) return 1;
+    for (i = 0; i < n; i++) {
+        if (fscanf(f, " %lf", &out[i]) != 1) return 1;
+        if (i < n - 1 && fscanf(f, " ,") < 0) return 1;
+    }
+    return 0;
+}
+
+/* fread_integral_area (evaluation_state.c:68-95): step sizes are recomputed
+ * from the geometry as read. */
+static int read_area(FILE* f, INTEGRAL_AREA* a)
+{
+    if (fscanf(f, " mu[min,max,steps] : %lf , %lf , %d", &a->mu_min, &a->mu_max, &a->mu_steps) != 3 ||
+        fscanf(f, " nu[min,max,steps] : %lf , %lf , %d", &a->nu_min, &a->nu_max, &a->nu_steps) != 3 ||
+        fscanf(f, " r[min,max,steps] : %lf , %lf , %d", &a->r_min, &a->r_max, &a->r_steps) != 3)
+        return 1;
+    if (a->mu_steps < 1 || a->nu_steps < 1 || a->r_steps < 1) return 1;
+    a->mu_step_size = (a->mu_max - a->mu_min) / a->mu_steps;
+    a->nu_step_size = (a->nu_max - a->nu_min) / a->nu_steps;
+    a->r_step_size = (a->r_max - a->r_min) / a->r_steps;
+    if (fscanf(f, " min_calculation : %ld , max_calculation : %ld , current_calculation : %ld",
+               &a->min_calculation, &a->max_calculation, &a->current_calculation) != 3)
+        return 1;
+    if (fscanf(f, " background_integral : %lf", &a->background_integral) != 1) return 1;
+    return read_fixed_array(f, "stream_integrals", a->number_streams, a->stream_integrals);
+}
+
+int mw_read_checkpoint(const char* filename, EVALUATION_STATE* es)
+{
+    FILE* f;
+    EVALUATION_STATE t;
+    INTEGRAL_AREA* areas = NULL;
+    double* totals = NULL;
+    int i, n_file = -1, n_parsed = 0, rc = 1;
+    if (!filename || !es) return 1;
+    f = fopen(filename, "r");
+    if (!f) return 0;
+    /* Parse into copies, so that a bad file leaves es as it was. */
+    t = *es;
+    totals = (double*)calloc(es->number_streams > 0 ? (size_t)es->number_streams : 1, sizeof(double));
+    areas = (INTEGRAL_AREA*)calloc(es->number_integrals > 0 ? (size_t)es->number_integrals : 1,
+                                   sizeof(INTEGRAL_AREA));
+    if (!totals || !areas) goto out;
+    if (fscanf(f, " background_integral : %lf", &t.background_integral) != 1) goto out;
+    if (read_fixed_array(f, "stream_integrals", es->number_streams, totals)) goto out;
+    if (fscanf(f, " prob_sum : %lf , num_zero : %d , bad_jacobians : %d", &t.prob_sum, &t.num_zero,
+               &t.bad_jacobians) != 3 ||
+        fscanf(f, " current_star_point : %d", &t.current_star_point) != 1 ||
+        fscanf(f, " current_integral : %d", &t.current_integral) != 1 ||
+        fscanf(f, " number_integrals : %d", &n_file) != 1 || n_file != es->number_integrals)
+        goto out;
+    for (i = 0; i < es->number_integrals; i++) {
+        areas[i] = *es->integral[i];
+        areas[i].stream_integrals = (double*)calloc(
+            areas[i].number_streams > 0 ? (size_t)areas[i].number_streams : 1, sizeof(double));
+        if (!areas[i].stream_integrals) goto out;
+        n_parsed = i + 1;
+        if (read_area(f, &areas[i])) goto out;
+    }
+    /* Commit: values into es's own arrays. */
+    es->background_integral = t.background_integral;
+    for (i = 0; i < es->number_streams; i++) es->stream_integrals[i] = totals[i];
+    es->prob_sum = t.prob_sum;
+    es->num_zero = t.num_zero;
+    es->bad_jacobians = t.bad_jacobians;
+    es->current_star_point = t.current_star_point;
+    es->current_integral = t.current_integral;
+    for (i = 0; i < es->number_integrals; i++) {
+        INTEGRAL_AREA* a = es->integral[i];
+        double* parsed = areas[i].stream_integrals;
+        double* keep = a->stream_integrals;
+        int s;
+        for (s = 0; s < a->number_streams; s++) keep[s] = parsed[s];
+        *a = areas[i];
+        a->stream_integrals = keep;
+    }
+    rc = 0;
+out:
+    for (i = 0; i < n_parsed; i++) free(areas[i].stream_integrals);
+    free(areas);
+    free(totals);
+    fclose(f);
+    return rc;
+}
+
+double mw_calculate_progress(const EVALUATION_STATE* es)
+{
+    double total = 0.0, done = 0.0;
+    int i;
+    for (i = 0; i < es->number_integrals; i++) {
+        const INTEGRAL_AREA* a = es->integral[i];
+        const double cells = (double)a->r_steps * a->mu_steps * a->nu_steps;
+        total += cells;
+        if (i < es->current_integral) {
+            done += cells;
+        } else if (i == es->current_integral) {
+            /* get_steps (evaluation_state.c:129-151): r fastest, then nu, then mu */
+            const long c = a->current_calculation;
+            const long r = c % a->r_steps;
+            const long nu = (c / a->r_steps) % a->nu_steps;
+            const long mu = c / ((long)a->r_steps * a->nu_steps);
+            done += r + (double)nu * a->r_steps + (double)mu * a->nu_steps * a->r_steps;
+        }
+    }
+    total += es->total_stars;
+    done += es->current_star_point;
+    return done / total;
+}
+
 int mw_get_optimized_parameter_count(const ASTRONOMY_PARAMETERS* ap)
 {
     int i, j, count = 0;

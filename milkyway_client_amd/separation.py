@@ -47,7 +47,16 @@ EXPORTS = [
     "mw_get_optimized_parameter_count", "mw_set_astronomy_parameters",
     "mw_write_astronomy_parameters", "mw_write_star_points",
     "mw_ctx_separation", "mw_hip_separation",
+    "mw_hip_integral_f", "mw_hip_likelihood_f", "mw_integral_compose", "mw_likelihood_compose",
+    "mw_split_astronomy_parameters", "mw_split_star_points", "mw_get_search_parameters",
+    "mw_get_step", "mw_get_min_parameters", "mw_get_max_parameters", "mw_hessian",
+    "mw_ctx_hessian", "mw_matrix_invert", "mw_errors",
+    "mw_write_checkpoint", "mw_read_checkpoint", "mw_calculate_progress",
 ]
+
+# int (*)(void* user, const double* vectors, int count, double* values)
+VECTOR_EVAL_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_double), C.c_int,
+                             C.POINTER(C.c_double))
 
 
 class MwError(RuntimeError):
@@ -194,6 +203,36 @@ def load_library() -> C.CDLL:
     L.mw_ctx_separation.restype = C.c_int
     L.mw_hip_separation.argtypes = [C.c_char_p, C.c_double, _dp, AP, SP, C.c_ulong]
     L.mw_hip_separation.restype = C.c_int
+    L.mw_hip_integral_f.argtypes = [_dp, _dp, AP, ES, SP]
+    L.mw_hip_integral_f.restype = None
+    L.mw_hip_likelihood_f.argtypes = [_dp, _dp, AP, ES, SP]
+    L.mw_hip_likelihood_f.restype = None
+    L.mw_integral_compose.argtypes = [_dp, C.c_int, _dp, AP]
+    L.mw_integral_compose.restype = None
+    L.mw_likelihood_compose.argtypes = [_dp, C.c_size_t, C.c_int]
+    L.mw_likelihood_compose.restype = C.c_double
+    L.mw_split_astronomy_parameters.argtypes = [AP, C.c_int, C.c_int]
+    L.mw_split_astronomy_parameters.restype = None
+    L.mw_split_star_points.argtypes = [SP, C.c_int, C.c_int]
+    L.mw_split_star_points.restype = C.c_int
+    for name in ("mw_get_search_parameters", "mw_get_step", "mw_get_min_parameters",
+                 "mw_get_max_parameters"):
+        getattr(L, name).argtypes = [AP, _dp]
+        getattr(L, name).restype = None
+    L.mw_hessian.argtypes = [C.c_int, _dp, _dp, VECTOR_EVAL_FN, C.c_void_p, _dp, _dp, _dp]
+    L.mw_hessian.restype = C.c_int
+    L.mw_ctx_hessian.argtypes = [C.c_void_p, _dp, _dp, C.c_int, _dp, _dp, _dp]
+    L.mw_ctx_hessian.restype = C.c_int
+    L.mw_matrix_invert.argtypes = [C.c_int, _dp, _dp]
+    L.mw_matrix_invert.restype = C.c_int
+    L.mw_errors.argtypes = [C.c_int, _dp, C.c_int, _dp, _dp]
+    L.mw_errors.restype = C.c_int
+    L.mw_write_checkpoint.argtypes = [C.c_char_p, ES]
+    L.mw_write_checkpoint.restype = C.c_int
+    L.mw_read_checkpoint.argtypes = [C.c_char_p, ES]
+    L.mw_read_checkpoint.restype = C.c_int
+    L.mw_calculate_progress.argtypes = [ES]
+    L.mw_calculate_progress.restype = C.c_double
     _lib = L
     return L
 
@@ -320,6 +359,22 @@ class Context:
         t = mw_timing()
         _check(load_library().mw_ctx_last_timing(self.handle, C.byref(t)))
         return t
+
+    def hessian(self, point, step, precision: int = PREC_FP64):
+        """Central-difference Hessian, gradient and value of the client's
+        likelihood (mw_ctx_hessian), all 2 n^2 + 1 points in batched launches."""
+        n = self.n_params
+        x = np.ascontiguousarray(point, dtype=np.float64)
+        h = np.ascontiguousarray(step, dtype=np.float64)
+        if x.size != n or h.size != n:
+            raise ValueError(f"point and step need {n} values")
+        H = np.zeros((n, n))
+        g = np.zeros(n)
+        v = C.c_double()
+        _check(load_library().mw_ctx_hessian(self.handle, x.ctypes.data_as(_dp), h.ctypes.data_as(_dp),
+                                             precision, H.ctypes.data_as(_dp), g.ctypes.data_as(_dp),
+                                             C.byref(v)))
+        return H, g, v.value
 
     def separation(self, vector=None, precision: int = PREC_FP64, integrals=None, seed: int = 0):
         """Stream separation (mw_ctx_separation): per-star assignment [N],
@@ -472,3 +527,15 @@ class Server:
 
     def __exit__(self, *a):
         self.close()
+
+
+def errors(hessian, number_stars: int):
+    """errors.c:38-84: sigma = H^-1 / number_stars and the standard deviations."""
+    H = np.ascontiguousarray(hessian, dtype=np.float64)
+    n = H.shape[0]
+    sigma = np.zeros((n, n))
+    sd = np.zeros(n)
+    if load_library().mw_errors(n, H.ctypes.data_as(_dp), number_stars, sigma.ctypes.data_as(_dp),
+                                sd.ctypes.data_as(_dp)):
+        raise np.linalg.LinAlgError("singular Hessian")
+    return sigma, sd
