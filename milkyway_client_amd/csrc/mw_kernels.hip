@@ -149,35 +149,74 @@ __device__ __forceinline__ double rcp_nr(double d)
 }
 
 // ---------------------------------------------------------------------------
-// One line of sight: its unit vector u and, per stream, the part of u
-// perpendicular to the stream axis scaled by k = sqrt(log2(e) / (2 sigma^2)).
-// A point at distance r along the sight sits at r u - (8.5, 0, 0); its offset
-// from the stream axis is r u_perp - c_perp (evaluation_optimized.c:283-293
-// computes the same vector as v - (v . a) a), so |r u_perp k - c_perp k|^2 is
-// the Gaussian exponent |v_perp|^2 / (2 sigma^2) in base 2: three FMAs and a
-// dot product per point, and the density is 2 to the minus that.
+// One line of sight.  Along the sight a point at distance r sits at
+// r u - (8.5, 0, 0), and both squared distances the integrand needs are
+// quadratics in r, written per sight in the shifted form (p r - m)^2 + e:
+//
+//  * background: rg^2 = (r u_x - 8.5)^2 + (r u_y)^2 + (r u_z / q)^2 with
+//    P = u_x^2 + u_y^2 + u_z^2 / q^2, p = sqrt(P), m = 8.5 u_x / p,
+//    e = 8.5^2 (u_y^2 + u_z^2 / q^2) / P (the minimum, with no cancellation);
+//  * stream s: the offset from the axis is r u_perp - c_perp, where
+//    x_perp = x - (x . a) a and c' = c + (8.5, 0, 0)
+//    (evaluation_optimized.c:283-293 computes the same vector as v - (v . a) a).
+//    Both are scaled by k = sqrt(log2(e) / (2 sigma^2)), so that
+//    |r u_perp k - c_perp k|^2 is the Gaussian exponent in base 2; with
+//    p = |u_perp k|, m = (u_perp k / p) . c_perp k and e the squared distance
+//    of c_perp k from the line through u_perp k (again a norm, not a
+//    difference of squares).
+//
+// Per convolution point each quadratic is then t = p r - m, t^2 + e: two
+// FMAs.  Near the minimum t is small and e carries the value, so the form
+// loses nothing to cancellation.
 // ---------------------------------------------------------------------------
-template <int S>
-struct Sight {
-    double ux, uy, uz;
-    double up[S > 0 ? S : 1][3];
+struct Quad {
+    double p, m, e;
 };
 
 template <int S>
+struct Sight {
+    Quad bg;
+    Quad st[S > 0 ? S : 1];
+};
+
+template <int S, int BG>
 __device__ __forceinline__ Sight<S> make_sight(double ux, double uy, double uz, const EvalConst& K)
 {
     Sight<S> v;
-    v.ux = ux;
-    v.uy = uy;
-    v.uz = uz;
+    if constexpr (BG != kBgZeroQ) {
+        const double side = fma(uy, uy, uz * uz * K.inv_q2);
+        const double P = fma(ux, ux, side);
+        const double p = sqrt(P);
+        v.bg = Quad{ p, kSunR0 * ux / p, (kSunR0 * kSunR0) * side / P };
+    } else {
+        v.bg = Quad{ 0.0, 0.0, 0.0 };
+    }
 #pragma unroll
     for (int s = 0; s < S; s++) {
         const double d = ux * K.sa[s][0] + uy * K.sa[s][1] + uz * K.sa[s][2];
-        v.up[s][0] = (ux - d * K.sa[s][0]) * K.ks[s];
-        v.up[s][1] = (uy - d * K.sa[s][1]) * K.ks[s];
-        v.up[s][2] = (uz - d * K.sa[s][2]) * K.ks[s];
+        const double w0 = (ux - d * K.sa[s][0]) * K.ks[s];
+        const double w1 = (uy - d * K.sa[s][1]) * K.ks[s];
+        const double w2 = (uz - d * K.sa[s][2]) * K.ks[s];
+        const double ww = fma(w0, w0, fma(w1, w1, w2 * w2));
+        const double c0 = K.cperp[s][0], c1 = K.cperp[s][1], c2 = K.cperp[s][2];
+        if (ww > 1e-200) {
+            const double p = sqrt(ww);
+            const double h0 = w0 / p, h1 = w1 / p, h2 = w2 / p;
+            const double m = fma(h0, c0, fma(h1, c1, h2 * c2));
+            const double e0 = fma(-m, h0, c0), e1 = fma(-m, h1, c1), e2 = fma(-m, h2, c2);
+            v.st[s] = Quad{ p, m, fma(e0, e0, fma(e1, e1, e2 * e2)) };
+        } else {
+            // sight parallel to the stream axis: constant distance
+            v.st[s] = Quad{ 0.0, 0.0, fma(c0, c0, fma(c1, c1, c2 * c2)) };
+        }
     }
     return v;
+}
+
+__device__ __forceinline__ double quad_at(const Quad& q, double r)
+{
+    const double t = fma(q.p, r, -q.m);
+    return fma(t, t, q.e);
 }
 
 // ---------------------------------------------------------------------------
@@ -190,10 +229,7 @@ __device__ __forceinline__ void point_terms(double r, double qw, const Sight<S>&
                                             const EvalConst& K, double& bg, double (&st)[kMaxStreams])
 {
     if constexpr (BG != kBgZeroQ) {
-        const double x = fma(r, v.ux, -kSunR0);
-        const double y = r * v.uy;
-        const double z = r * v.uz;
-        const double rg2 = fma(x, x, fma(y, y, z * z * K.inv_q2));
+        const double rg2 = quad_at(v.bg, r);
         if constexpr (MIXED) {
             // FP32 estimates of 1/sqrt and 1/x, one FP64 Newton step each
             double rinv = (double)__builtin_amdgcn_rsqf((float)rg2);
@@ -213,10 +249,7 @@ __device__ __forceinline__ void point_terms(double r, double qw, const Sight<S>&
     }
 #pragma unroll
     for (int s = 0; s < S; s++) {
-        const double w0 = fma(r, v.up[s][0], -K.cperp[s][0]);
-        const double w1 = fma(r, v.up[s][1], -K.cperp[s][1]);
-        const double w2 = fma(r, v.up[s][2], -K.cperp[s][2]);
-        const double a = fma(w0, w0, fma(w1, w1, w2 * w2));
+        const double a = quad_at(v.st[s], r);
         if constexpr (MIXED) {
             // 2^t with t = -a split in FP64 into n + f, f in [0, 1): only 2^f
             // goes through FP32, so the relative error stays at the FP32
@@ -257,7 +290,7 @@ __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
     const long last = (col0 + kBlock < C ? col0 + kBlock : C);
     const bool active = it.calc_end > col0 * R && it.calc_begin < last * R;
     if (active) {
-        const Sight<S> sight = make_sight<S>(it.col[cc], it.col[C + cc], it.col[2 * C + cc], K);
+        const Sight<S> sight = make_sight<S, BG>(it.col[cc], it.col[C + cc], it.col[2 * C + cc], K);
         const double wnu = it.col[3 * C + cc];
         const long base = cc * (long)R;
         long lo_l = it.calc_begin - base, hi_l = it.calc_end - base;
@@ -396,7 +429,7 @@ __device__ __forceinline__ void star_densities(const double* __restrict__ star, 
                                                const double* __restrict__ conv_kw, double& reff,
                                                double& bg, double (&st)[kMaxStreams])
 {
-    const Sight<S> sight = make_sight<S>(star[i], star[N + i], star[2 * N + i], K);
+    const Sight<S> sight = make_sight<S, BG>(star[i], star[N + i], star[2 * N + i], K);
     const double rstar = star[3 * N + i];
     reff = star[4 * N + i];
 
