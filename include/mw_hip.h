@@ -41,6 +41,27 @@ extern "C" {
 /* Precision modes */
 #define MW_PREC_FP64 0      /* FP64 throughout */
 #define MW_PREC_MIXED 1     /* FP32 sqrt/reciprocal/exp, FP64 positions, distances, sums, logs */
+/* MW_PREC_MIXED on a resampled quadrature (context, batch and server calls;
+ * not the drop-in entry points).  The reference's integral sums, per sight,
+ * r_steps x convolve weighted densities at distances that are the same for
+ * every sight.  Here those weights are folded, by 16-node Lagrange
+ * interpolation, onto a uniform grid in magnitude whose spacing follows the
+ * narrowest feature the model can have (the thinnest stream at the farthest
+ * distance, the closest approach to the Galactic centre), and the sum is
+ * taken over the grid: the same quadrature to about 1e-10 per sight, from up
+ * to 40-80x fewer density evaluations on large wedges.  Regions where it
+ * would not pay, and background modes other than the Hernquist profile,
+ * use the exact tables.  The star pass is the exact one. */
+#define MW_PREC_RESAMPLED 2
+
+/* The weight folding behind MW_PREC_RESAMPLED, on the host, for any point
+ * set: points g[n] (magnitudes) with weights w[n] become weights K[m] on the
+ * grid *g_lo + m delta, so that sum_m K[m] f(grid_m) equals
+ * sum_p w[p] f(g[p]) for every polynomial f of the given degree, and to the
+ * interpolation error otherwise.  Returns the grid size M (K is written
+ * only when capacity >= M), or -1 for bad input. */
+long mw_resample_weights(const double* g, const double* w, long n, double delta, int degree,
+                         double* g_lo, double* K, long capacity);
 
 /* ------------------------------------------------------------------------
  * Drop-in entry points

@@ -58,6 +58,7 @@ struct Plan {
     std::vector<EvalConst> K;              // one per slot
     std::vector<PlanRegion> regions;       // slot-major, region order within a slot
     bool integrals = false, stars = false, mixed = false, device_finalize = false;
+    bool resample = false;                 // resampled quadrature where it pays (whole regions)
     long star_begin = 0;
     double* log10p_host = nullptr;         // [n_slots][n_stars]
 };
@@ -93,6 +94,23 @@ int build_context(mw_ctx* c, const std::vector<RegionGeom>& geoms, int n_conv, i
         MW_HIP_CHECK(hipMemcpy(rd->rq.p, t.rq.data(), sizeof(double) * t.rq.size(), hipMemcpyHostToDevice));
         MW_HIP_CHECK(hipMemcpy(rd->rstep.p, t.rstep.data(), sizeof(RStep) * t.rstep.size(), hipMemcpyHostToDevice));
         MW_HIP_CHECK(hipMemcpy(rd->col.p, t.col.data(), sizeof(double) * t.col.size(), hipMemcpyHostToDevice));
+        // What the resampled quadrature needs: the exact tables on the host
+        // and the region's extremes.
+        double gmin = INFINITY, gmax = -INFINITY;
+        for (size_t i = 0; i < t.rq.size(); i += 2) {
+            const double r = t.rq[i];
+            rd->r_max = std::max(rd->r_max, r);
+            const double gm = 5.0 * (std::log10(r * 1000.0) - 1.0) + kAbsM;
+            gmin = std::min(gmin, gm);
+            gmax = std::max(gmax, gm);
+        }
+        rd->g_span = gmax - gmin;
+        for (long col = 0; col < t.n_cols; col++) {
+            const double ux = t.col[col];
+            rd->min_sin_gc = std::min(rd->min_sin_gc, std::sqrt(std::max(0.0, 1.0 - ux * ux)));
+        }
+        rd->h_rq = std::move(t.rq);
+        rd->h_rstep = std::move(t.rstep);
         c->regions.push_back(std::move(rd));
     }
     c->n_stars = sp ? sp->number_stars : 0;
@@ -148,16 +166,25 @@ int run_plan(mw_ctx* c, Plan& P)
             if (igroups.empty() || igroups.back().bg != bg)
                 igroups.push_back(Group{ bg, item, 0, total_iblocks, 0 });
             Group& g = igroups.back();
-            const RegionDev& rd = *c->regions[pr.region];
+            RegionDev& rd = *c->regions[pr.region];
+            TableRef tr;
+            const int level = select_level(rd, P.K[pr.slot], P.resample);
+            if ((rc = region_tables(rd, level, &tr))) return rc;
             RegionLaunch& L = c->h_ritems.p[item];
-            L.rq = rd.rq.p;
-            L.rstep = rd.rstep.p;
+            L.rq = tr.rq;
+            L.rstep = tr.rstep;
             L.col = rd.col.p;
             L.n_cols = rd.n_cols;
-            L.r_steps = rd.geom.r_steps;
-            L.n_conv = rd.n_conv;
+            L.r_steps = tr.r_steps;
+            L.n_conv = tr.n_conv;
             L.calc_begin = pr.begin;
             L.calc_end = pr.end;
+            if (level >= 0) {
+                // A resampled region is always evaluated whole; its flat
+                // index space is columns x grid rows.
+                L.calc_begin = 0;
+                L.calc_end = rd.n_cols * (long)tr.r_steps;
+            }
             L.slot = pr.slot;
             L.out_index = order[oi];
             L.block_begin = g.n_blocks;          // block index within the group's launch
@@ -431,8 +458,12 @@ size_t mw_ctx_resident_bytes(const mw_ctx* c)
 {
     if (!c) return 0;
     size_t b = 0;
-    for (const auto& r : c->regions)
+    for (const auto& r : c->regions) {
         b += r->rq.cap * sizeof(double) + r->rstep.cap * sizeof(RStep) + r->col.cap * sizeof(double);
+        std::lock_guard<std::mutex> lock(r->level_mu);
+        for (const auto& q : r->levels)
+            if (q) b += q->rq.cap * sizeof(double) + q->rstep.cap * sizeof(RStep);
+    }
     b += (c->star.cap + c->ratio.cap + c->kw.cap) * sizeof(double);
     return b;
 }
@@ -442,7 +473,8 @@ int mw_ctx_eval(mw_ctx* c, const double* params, int n_vec, int precision, mw_re
 {
     if (!c || n_vec < 1 || !results) return fail(MW_E_INVALID, "bad arguments");
     if (c->n_params > 0 && !params) return fail(MW_E_INVALID, "null parameter vector");
-    if (precision != MW_PREC_FP64 && precision != MW_PREC_MIXED) return fail(MW_E_INVALID, "unknown precision");
+    if (precision != MW_PREC_FP64 && precision != MW_PREC_MIXED && precision != MW_PREC_RESAMPLED)
+        return fail(MW_E_INVALID, "unknown precision");
     const int G = (int)c->regions.size();
     Plan P;
     P.K.resize(n_vec);
@@ -459,7 +491,8 @@ int mw_ctx_eval(mw_ctx* c, const double* params, int n_vec, int precision, mw_re
     }
     P.integrals = true;
     P.stars = c->n_stars > 0;
-    P.mixed = precision == MW_PREC_MIXED;
+    P.mixed = precision != MW_PREC_FP64;
+    P.resample = precision == MW_PREC_RESAMPLED;
     P.device_finalize = true;
     P.star_begin = 0;
     P.log10p_host = log10p;

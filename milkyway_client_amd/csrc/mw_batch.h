@@ -37,7 +37,8 @@ public:
     // Builds the work items for these slots (same device as init).  Must be
     // called again when the slots, their contexts or their background modes
     // change; keeps constants in the pinned staging buffer.
-    int plan(const std::vector<Slot>& slots, bool mixed);
+    // precision: MW_PREC_FP64, MW_PREC_MIXED or MW_PREC_RESAMPLED.
+    int plan(const std::vector<Slot>& slots, int precision);
     // Rewrites the staged constants of the planned slots (same modes).
     void set_constants(const std::vector<Slot>& slots);
     bool same_modes(const std::vector<Slot>& slots) const;
@@ -69,8 +70,9 @@ private:
     int device_ = 0, flags_ = 0, n_cu_ = 256;
     hipStream_t stream_ = nullptr;
     hipEvent_t ev_[4] = {};
-    bool mixed_ = false;
+    bool mixed_ = false, resample_ = false;
     int variant_ = 0, blocks_per_cu_ = 2;
+    std::vector<int> levels_;        // quadrature level per (slot, region), slot-major
     std::vector<int> slot_bg_, slot_n_regions_, slot_first_row_, slot_S_;
     std::vector<Group> igroups_, sgroups_;
     int n_ritems_ = 0, total_iblocks_ = 0, total_sblocks_ = 0;

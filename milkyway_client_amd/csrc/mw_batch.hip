@@ -38,9 +38,14 @@ int BatchRunner::init(int device, int flags)
 bool BatchRunner::same_modes(const std::vector<Slot>& slots) const
 {
     if (slots.size() != slot_bg_.size()) return false;
-    for (size_t s = 0; s < slots.size(); s++)
+    size_t li = 0;
+    for (size_t s = 0; s < slots.size(); s++) {
         if (slots[s].k.bg_mode != slot_bg_[s]) return false;
-    return true;
+        // The planned work items point at one quadrature level per region.
+        for (const auto& rd : slots[s].ctx->regions)
+            if (li >= levels_.size() || select_level(*rd, slots[s].k, resample_) != levels_[li++]) return false;
+    }
+    return li == levels_.size();
 }
 
 size_t BatchRunner::scratch_bytes_for(const std::vector<Slot>& slots)
@@ -64,12 +69,14 @@ size_t BatchRunner::scratch_bytes() const
            (first_.cap + count_.cap) * sizeof(int);
 }
 
-int BatchRunner::plan(const std::vector<Slot>& slots, bool mixed)
+int BatchRunner::plan(const std::vector<Slot>& slots, int precision)
 {
     const int NI = integral_comps(), NS = star_comps();
     const int n = (int)slots.size();
     if (n < 1) return fail(MW_E_INVALID, "empty batch");
-    mixed_ = mixed;
+    mixed_ = precision != MW_PREC_FP64;
+    resample_ = precision == MW_PREC_RESAMPLED;
+    levels_.clear();
     slot_bg_.assign(n, 0);
     slot_n_regions_.assign(n, 0);
     slot_first_row_.assign(n, 0);
@@ -92,16 +99,21 @@ int BatchRunner::plan(const std::vector<Slot>& slots, bool mixed)
         slot_n_regions_[s] = (int)c->regions.size();
         slot_stars_[s] = c->n_stars;
         for (const auto& rd : c->regions) {
+            TableRef tr;
+            const int level = select_level(*rd, slots[s].k, resample_);
+            int rc0 = region_tables(*rd, level, &tr);
+            if (rc0) return rc0;
+            levels_.push_back(level);
             RegionLaunch L;
             std::memset(&L, 0, sizeof(L));
-            L.rq = rd->rq.p;
-            L.rstep = rd->rstep.p;
+            L.rq = tr.rq;
+            L.rstep = tr.rstep;
             L.col = rd->col.p;
             L.n_cols = rd->n_cols;
-            L.r_steps = rd->geom.r_steps;
-            L.n_conv = rd->n_conv;
+            L.r_steps = tr.r_steps;
+            L.n_conv = tr.n_conv;
             L.calc_begin = 0;
-            L.calc_end = (long)rd->geom.r_steps * rd->geom.mu_steps * rd->geom.nu_steps;
+            L.calc_end = rd->n_cols * (long)tr.r_steps;
             L.slot = s;
             L.out_index = row++;
             L.block_count = rd->blocks;
@@ -316,7 +328,7 @@ int slot_constants(mw_ctx* c, const double* vec, EvalConst* k)
 struct mw_batch {
     int device = 0;
     int flags = 0;
-    bool mixed = false;
+    int precision = MW_PREC_FP64;
     std::vector<mw_ctx*> ctxs;
     std::vector<int> nvec;
     std::vector<Slot> slots;
@@ -344,11 +356,12 @@ int mw_batch_create(mw_ctx* const* ctxs, const int* n_vectors, int n_ctxs, int p
                     mw_batch** out)
 {
     if (!ctxs || !n_vectors || n_ctxs < 1 || !out) return fail(MW_E_INVALID, "bad arguments");
-    if (precision != MW_PREC_FP64 && precision != MW_PREC_MIXED) return fail(MW_E_INVALID, "unknown precision");
+    if (precision != MW_PREC_FP64 && precision != MW_PREC_MIXED && precision != MW_PREC_RESAMPLED)
+        return fail(MW_E_INVALID, "unknown precision");
     *out = nullptr;
     auto b = std::make_unique<mw_batch>();
     b->flags = flags;
-    b->mixed = precision == MW_PREC_MIXED;
+    b->precision = precision;
     int total = 0;
     for (int i = 0; i < n_ctxs; i++) {
         if (!ctxs[i] || n_vectors[i] < 0) return fail(MW_E_INVALID, "bad context or vector count");
@@ -403,7 +416,7 @@ int mw_batch_run(mw_batch* b, const double* params, mw_result* results)
     }
     int rc;
     if (!b->planned || !b->runner.same_modes(slots)) {
-        if ((rc = b->runner.plan(slots, b->mixed))) return rc;
+        if ((rc = b->runner.plan(slots, b->precision))) return rc;
         b->planned = true;
         b->drop_graph();
     } else {

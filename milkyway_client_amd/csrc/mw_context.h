@@ -9,6 +9,7 @@
 #include <cstdint>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -242,6 +243,15 @@ inline void reset_state(EVALUATION_STATE* es)
     }
 }
 
+// Resampled quadrature tables of one region at one grid spacing.
+constexpr int kQuadLevels = 13;   // grid spacing 0.02 * 2^(-L/2) magnitudes, L < kQuadLevels
+struct ResampledDev {
+    int r_steps = 0, n_conv = 0;
+    long nodes = 0;
+    DevBuf<double> rq;
+    DevBuf<RStep> rstep;
+};
+
 struct RegionDev {
     RegionGeom geom;
     int n_conv = 0;
@@ -249,7 +259,32 @@ struct RegionDev {
     int blocks = 0;
     DevBuf<double> rq, col;
     DevBuf<RStep> rstep;
+    // Host copies of the exact r-step tables, and what the choice of a
+    // resampled grid needs: the largest convolution distance (kpc), the
+    // magnitude span of the convolution points, and the smallest
+    // sqrt(1 - u_x^2) over the region's sights (their closest approach to
+    // the Galactic centre is 8.5 kpc times that for q = 1).
+    std::vector<double> h_rq;
+    std::vector<RStep> h_rstep;
+    double r_max = 0.0, g_span = 0.0, min_sin_gc = 1.0;
+    std::unique_ptr<ResampledDev> levels[kQuadLevels];
+    std::mutex level_mu;
 };
+
+// Tables a launch reads for one region: the exact ones or a resampled level.
+struct TableRef {
+    const double* rq;
+    const RStep* rstep;
+    int r_steps, n_conv;
+    int level;   // -1 for the exact tables
+};
+
+// Resampled level to use for region rd under the model k, or -1 for the
+// exact tables (resampling off, a background mode it does not cover, or no
+// gain).  Pure: builds nothing.
+int select_level(const RegionDev& rd, const EvalConst& k, bool resample);
+// The tables of that level, built and uploaded on first use.
+int region_tables(RegionDev& rd, int level, TableRef* out);
 
 }  // namespace mw
 

@@ -35,7 +35,7 @@ double now_s()
 struct Request {
     long id;
     int wedge;
-    bool mixed;
+    int precision;
     EvalConst k;
     double t_submit;
 };
@@ -89,7 +89,7 @@ bool mw_server::launch_locked(InFlight& f, std::unique_lock<std::mutex>& lk)
     const size_t budget = cfg.hbm_budget_bytes;
     while (!pending.empty() && (int)take.size() < cfg.max_batch_vectors) {
         const Request& r = pending.front();
-        if (!take.empty() && r.mixed != take.front().mixed) break;
+        if (!take.empty() && r.precision != take.front().precision) break;
         std::vector<Slot> trial = slots_v;
         trial.push_back(Slot{ wedges[r.wedge].get(), r.k });
         const size_t need = BatchRunner::scratch_bytes_for(trial);
@@ -107,10 +107,10 @@ bool mw_server::launch_locked(InFlight& f, std::unique_lock<std::mutex>& lk)
     inflight_scratch += scratch;
     n_batches++;
     max_batch_seen = std::max<long>(max_batch_seen, (long)f.reqs.size());
-    const bool mixed = f.reqs.front().mixed;
+    const int precision = f.reqs.front().precision;
     lk.unlock();
     int rc = hipSetDevice(device) == hipSuccess ? MW_OK : MW_E_HIP;
-    if (!rc) rc = f.runner->plan(slots_v, mixed);
+    if (!rc) rc = f.runner->plan(slots_v, precision);
     if (!rc) rc = f.runner->enqueue();
     lk.lock();
     if (rc) {
@@ -238,7 +238,8 @@ int mw_server_add_wedge(mw_server* s, const ASTRONOMY_PARAMETERS* ap, const STAR
 int mw_server_submit(mw_server* s, int wedge_id, const double* params, int precision, long* request_id)
 {
     if (!s || !request_id) return fail(MW_E_INVALID, "null argument");
-    if (precision != MW_PREC_FP64 && precision != MW_PREC_MIXED) return fail(MW_E_INVALID, "unknown precision");
+    if (precision != MW_PREC_FP64 && precision != MW_PREC_MIXED && precision != MW_PREC_RESAMPLED)
+        return fail(MW_E_INVALID, "unknown precision");
     mw_ctx* c;
     {
         std::lock_guard<std::mutex> lk(s->mu);
@@ -248,7 +249,7 @@ int mw_server_submit(mw_server* s, int wedge_id, const double* params, int preci
     if (c->n_params > 0 && !params) return fail(MW_E_INVALID, "null parameters");
     Request r;
     r.wedge = wedge_id;
-    r.mixed = precision == MW_PREC_MIXED;
+    r.precision = precision;
     int rc = slot_constants(c, params, &r.k);
     if (rc) return rc;
     // A request that could never fit the budget is refused now.

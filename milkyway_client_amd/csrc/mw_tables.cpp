@@ -14,6 +14,7 @@
 //                     atSurveyGeometry.c:414-460
 // Compiled with -ffp-contract=off so the host arithmetic rounds like the
 // reference's scalar SSE2 code.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <thread>
@@ -471,6 +472,100 @@ void star_conv_tables(int n, std::vector<double>* ratio, std::vector<double>* kw
     }
 }
 
+long resample_quadrature(const double* g, const double* w, long n, double delta, int degree,
+                         double* g_lo, std::vector<double>* K)
+{
+    if (n < 1 || !(delta > 0.0) || degree < 1 || degree > 31) return -1;
+    double gmin = g[0], gmax = g[0];
+    for (long i = 1; i < n; i++) {
+        gmin = std::min(gmin, g[i]);
+        gmax = std::max(gmax, g[i]);
+    }
+    const int half = degree / 2;
+    const double lo = gmin - (half + 1) * delta;
+    const long M = (long)std::ceil((gmax - lo) / delta) + half + 2;
+    // Barycentric weights of degree + 1 equally spaced nodes: (-1)^i C(D, i).
+    std::vector<long double> bw(degree + 1);
+    long double c = 1.0L;
+    for (int i = 0; i <= degree; i++) {
+        bw[i] = (i % 2 ? -c : c);
+        c = c * (degree - i) / (i + 1);
+    }
+    std::vector<long double> acc(M, 0.0L), lw(degree + 1);
+    for (long p = 0; p < n; p++) {
+        // Stencil of degree + 1 nodes around g[p], centred on its interval.
+        const long double x = ((long double)g[p] - lo) / delta;
+        long i0 = (long)std::floor((double)x) - (degree - 1) / 2;
+        if (i0 < 0) i0 = 0;
+        if (i0 > M - degree - 1) i0 = M - degree - 1;
+        int hit = -1;
+        long double den = 0.0L;
+        for (int i = 0; i <= degree; i++) {
+            const long double t = x - (long double)(i0 + i);
+            if (t == 0.0L) {
+                hit = i;
+                break;
+            }
+            lw[i] = bw[i] / t;
+            den += lw[i];
+        }
+        if (hit >= 0) {
+            acc[i0 + hit] += w[p];
+            continue;
+        }
+        for (int i = 0; i <= degree; i++) acc[i0 + i] += (long double)w[p] * (lw[i] / den);
+    }
+    K->assign(M, 0.0);
+    for (long m = 0; m < M; m++) (*K)[m] = (double)acc[m];
+    *g_lo = lo;
+    return M;
+}
+
+long build_resampled_tables(const std::vector<double>& rq, const std::vector<RStep>& rstep, int n_conv,
+                            double delta, RegionTables* out)
+{
+    const long R = (long)rstep.size();
+    const long N = R * n_conv;
+    std::vector<double> g(N), w(N);
+    double Q0 = 0, Q1 = 0, Q2 = 0;
+    for (long k = 0; k < R; k++) {
+        const double f = rstep[k].irv * rstep[k].reff_xr_rp3;
+        Q0 += f * rstep[k].q0;
+        Q1 += f * rstep[k].q1;
+        Q2 += f * rstep[k].q2;
+        for (long j = 0; j < n_conv; j++) {
+            const double r = rq[2 * (k * n_conv + j)];
+            g[k * n_conv + j] = 5.0 * (std::log10(r * 1000.0) - 1.0) + kAbsM;
+            w[k * n_conv + j] = f * rq[2 * (k * n_conv + j) + 1];
+        }
+    }
+    double lo;
+    std::vector<double> K;
+    const long M = resample_quadrature(g.data(), w.data(), N, delta, kResampleDegree, &lo, &K);
+    if (M < 1) return -1;
+    const int row = kResampleRow;
+    const long Rp = (M + row - 1) / row;
+    out->r_steps = (int)Rp;
+    out->n_conv = row;
+    out->rq.assign((size_t)2 * Rp * row, 0.0);
+    const double r_last = std::pow(10.0, (lo + (M - 1) * delta - kAbsM) / 5.0 + 1.0) / 1000.0;
+    for (long m = 0; m < Rp * row; m++) {
+        // r(g) = 10^((g - 4.2) / 5 + 1) / 1000; padding entries carry no weight.
+        out->rq[2 * m] = m < M ? std::pow(10.0, (lo + m * delta - kAbsM) / 5.0 + 1.0) / 1000.0 : r_last;
+        out->rq[2 * m + 1] = m < M ? K[m] : 0.0;
+    }
+    out->rstep.assign(Rp, RStep{});
+    for (long k = 0; k < Rp; k++) {
+        out->rstep[k].irv = 1.0;
+        out->rstep[k].reff_xr_rp3 = 1.0;
+    }
+    // The aux profile's per-point polynomial in g is exact as three moments.
+    out->rstep[0].q0 = Q0;
+    out->rstep[0].q1 = Q1;
+    out->rstep[0].q2 = Q2;
+    return M;
+}
+
 bool make_eval_const(const ASTRONOMY_PARAMETERS& ap, EvalConst* k, std::string* err)
 {
     std::memset(k, 0, sizeof(*k));
@@ -543,3 +638,14 @@ bool make_eval_const(const ASTRONOMY_PARAMETERS& ap, EvalConst* k, std::string* 
 }
 
 }  // namespace mw
+
+extern "C" long mw_resample_weights(const double* g, const double* w, long n, double delta, int degree,
+                                    double* g_lo, double* K, long capacity)
+{
+    if (!g || !w || !g_lo) return -1;
+    std::vector<double> k;
+    const long M = mw::resample_quadrature(g, w, n, delta, degree, g_lo, &k);
+    if (M < 1) return -1;
+    if (K && capacity >= M) std::memcpy(K, k.data(), sizeof(double) * M);
+    return M;
+}

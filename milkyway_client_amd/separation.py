@@ -28,7 +28,7 @@ LIB_PATH = Path(__file__).resolve().parent / "lib" / "libmw_separation_hip.so"
 CLIENT_PATH = LIB_PATH.parent / "milkyway_separation_hip"
 
 MW_OK, MW_E_INVALID, MW_E_HIP, MW_E_NOMEM, MW_E_NODEVICE = 0, 1, 2, 3, 4
-PREC_FP64, PREC_MIXED = 0, 1
+PREC_FP64, PREC_MIXED, PREC_RESAMPLED = 0, 1, 2
 MAX_STREAMS = 4
 CTX_TIMING = 1
 
@@ -52,6 +52,7 @@ EXPORTS = [
     "mw_get_step", "mw_get_min_parameters", "mw_get_max_parameters", "mw_hessian",
     "mw_ctx_hessian", "mw_matrix_invert", "mw_errors",
     "mw_write_checkpoint", "mw_read_checkpoint", "mw_calculate_progress",
+    "mw_resample_weights",
 ]
 
 # int (*)(void* user, const double* vectors, int count, double* values)
@@ -233,6 +234,8 @@ def load_library() -> C.CDLL:
     L.mw_read_checkpoint.restype = C.c_int
     L.mw_calculate_progress.argtypes = [ES]
     L.mw_calculate_progress.restype = C.c_double
+    L.mw_resample_weights.argtypes = [_dp, _dp, C.c_long, C.c_double, C.c_int, _dp, _dp, C.c_long]
+    L.mw_resample_weights.restype = C.c_long
     _lib = L
     return L
 
@@ -539,3 +542,19 @@ def errors(hessian, number_stars: int):
                                 sd.ctypes.data_as(_dp)):
         raise np.linalg.LinAlgError("singular Hessian")
     return sigma, sd
+
+
+def resample_weights(g, w, delta: float, degree: int = 15):
+    """mw_resample_weights: grid start, grid weights K."""
+    g = np.ascontiguousarray(g, dtype=np.float64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    lo = C.c_double()
+    L = load_library()
+    M = L.mw_resample_weights(g.ctypes.data_as(_dp), w.ctypes.data_as(_dp), g.size, delta, degree,
+                              C.byref(lo), None, 0)
+    if M < 1:
+        raise ValueError("bad resampling input")
+    K = np.zeros(M)
+    L.mw_resample_weights(g.ctypes.data_as(_dp), w.ctypes.data_as(_dp), g.size, delta, degree,
+                          C.byref(lo), K.ctypes.data_as(_dp), M)
+    return lo.value, K

@@ -1,0 +1,141 @@
+"""The resampled-quadrature mode (MW_PREC_RESAMPLED) on the GPU.
+
+It is mixed precision on a quadrature whose weights are folded onto a
+magnitude grid.  Bars, as for the mixed path: every star's |delta ln P| <= 1e-7
+against the FP64 oracle, integrals within 2e-8 relative (the FP32
+transcendentals dominate; the quadrature change itself is ~1e-10).  Cases
+include the narrowest streams the parameter ranges allow (sigma = 1 kpc),
+all four stream counts, the aux profile, the full-size wedge against the
+GPU's own FP64 result, and bitwise equality between single, batched
+(eager and graph) and served evaluations.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import fixture, rel
+from milkyway_client_amd import separation as sep
+from milkyway_client_amd import synth
+from milkyway_client_amd.refstructs import ParamsHolder, StarsHolder
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+LNP_TOL = 1e-7
+INT_REL_TOL = 2e-8
+LONG_R = synth.WedgeShape(700, 24, 16, 120, 2000)   # full r and convolution resolution
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    sep.load_library()
+    sep.require_device()
+
+
+def errors_vs_oracle(p, stars):
+    ph, sh = ParamsHolder(p), StarsHolder(stars)
+    with sep.Context(ph, sh) as ctx:
+        before = ctx.resident_bytes
+        got = ctx.eval(p.parameter_vector(), precision=sep.PREC_RESAMPLED, per_star=True)[0]
+        grew = ctx.resident_bytes > before
+    ref = O.evaluate_full(ParamsHolder(p), StarsHolder(stars))
+    e = dict(I_bg=rel(got.background_integral, ref.background_integral),
+             I_st=rel(got.stream_integrals, ref.stream_integrals) if p.number_streams else 0.0,
+             lnP=math.log(10) * np.max(np.abs(got.log10_p - ref.log10_p)),
+             lnL=math.log(10) * abs(got.likelihood - ref.likelihood))
+    print({k: f"{v:.2e}" for k, v in e.items()}, "resampled tables built:", grew)
+    return e, grew
+
+
+@pytest.mark.parametrize("S", [1, 2, 3, 4])
+def test_resampled_against_oracle(S):
+    p, stars = synth.workload(90 + S, LONG_R, S)
+    e, grew = errors_vs_oracle(p, stars)
+    assert grew, "the resampled grid was not used"
+    assert e["lnP"] <= LNP_TOL and e["lnL"] <= LNP_TOL
+    assert e["I_bg"] <= INT_REL_TOL and e["I_st"] <= INT_REL_TOL
+
+
+def test_narrowest_streams_far_out():
+    """sigma = 1 kpc (the parameter files' lower bound) for every stream."""
+    p, stars = synth.workload(97, LONG_R, 3)
+    for s in p.streams:
+        s.parameters[4] = 1.0
+    e, grew = errors_vs_oracle(p, stars)
+    assert grew
+    assert e["lnP"] <= LNP_TOL and e["I_st"] <= INT_REL_TOL and e["I_bg"] <= INT_REL_TOL
+
+
+def test_aux_profile_fixture():
+    p, stars = fixture("86", n_stars=1500, aux=True)
+    e, _ = errors_vs_oracle(p, stars)
+    assert e["lnP"] <= LNP_TOL and e["I_bg"] <= INT_REL_TOL and e["I_st"] <= INT_REL_TOL
+
+
+def test_full_size_against_gpu_fp64():
+    p, stars = synth.workload(0, synth.FULL_SIZE, synth.FULL_SIZE_STREAMS)
+    print("full size:", len(stars), "stars", flush=True)
+    ph, sh = ParamsHolder(p), StarsHolder(stars)
+    vec = p.parameter_vector()
+    with sep.Context(ph, sh, timing=True) as ctx:
+        ref = ctx.eval(vec, precision=sep.PREC_FP64, per_star=True)[0]
+        t64 = ctx.timing().integral_ms
+        got = ctx.eval(vec, precision=sep.PREC_RESAMPLED, per_star=True)[0]
+        trs = ctx.timing().integral_ms
+    d = dict(I_bg=rel(got.background_integral, ref.background_integral),
+             I_st=rel(got.stream_integrals, ref.stream_integrals),
+             lnP=math.log(10) * np.max(np.abs(got.log10_p - ref.log10_p)))
+    print({k: f"{v:.2e}" for k, v in d.items()}, f"integral ms: fp64 {t64:.1f} resampled {trs:.2f}")
+    assert d["lnP"] <= LNP_TOL and d["I_bg"] <= INT_REL_TOL and d["I_st"] <= INT_REL_TOL
+    assert trs < t64 / 10
+
+
+SHAPES = [  # (shape, streams, vectors): two large enough to resample, one not
+    (synth.WedgeShape(300, 20, 12, 60, 500), 2, 3),
+    (synth.WedgeShape(500, 16, 16, 120, 400, cut=(500, 4, 16, 330.0, 340.0)), 4, 2),
+    (synth.WedgeShape(20, 30, 16, 30, 300), 1, 2),
+]
+
+
+@pytest.fixture(scope="module")
+def wedges():
+    out = []
+    for k, (shape, S, nv) in enumerate(SHAPES):
+        p, stars = synth.workload(110 + k, shape, S)
+        ph, sh = ParamsHolder(p), StarsHolder(stars)
+        out.append((sep.Context(ph, sh), synth.parameter_vectors(110 + k, p, nv), ph, sh))
+    yield out
+    for ctx, *_ in out:
+        ctx.close()
+
+
+def same(a, b):
+    return (a.likelihood == b.likelihood and a.background_integral == b.background_integral
+            and np.array_equal(a.stream_integrals, b.stream_integrals) and a.num_zero == b.num_zero)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_equals_single(wedges, graph):
+    singles = [ctx.eval(v, precision=sep.PREC_RESAMPLED)[0] for ctx, vecs, *_ in wedges for v in vecs]
+    with sep.Batch([(c, len(v)) for c, v, *_ in wedges], precision=sep.PREC_RESAMPLED, graph=graph) as b:
+        for _ in range(2 if graph else 1):
+            got = b.run([v for _, v, *_ in wedges])
+            assert all(same(g, r) for g, r in zip(got, singles))
+
+
+def test_served_equals_single(wedges):
+    with sep.Server(max_inflight=2, max_batch=4) as srv:
+        ids = [srv.add_wedge(ph, sh) for _, _, ph, sh in wedges]
+        rids, want = [], []
+        for w, (ctx, vecs, *_) in enumerate(wedges):
+            for v in vecs:
+                rids.append(srv.submit(ids[w], v, sep.PREC_RESAMPLED))
+                want.append(ctx.eval(v, precision=sep.PREC_RESAMPLED)[0])
+        got = [srv.wait(r)[0] for r in rids]
+    assert all(same(g, r) for g, r in zip(got, want))
+
+
+def test_drop_in_does_not_take_resampled():
+    with pytest.raises(sep.MwError):
+        sep.set_precision(sep.PREC_RESAMPLED)
