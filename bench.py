@@ -2,18 +2,21 @@
 """Benchmark of the MI355X separation likelihood evaluation.
 
 Headline (default ``--workload batch``): batched mixed-precision likelihood
-evaluations against plain FP64.  A step is one batch of 64 parameter vectors
-(4 streams each, 16 per wedge) over 4 resident wedges of uneven size --
-70x160x64 (n=30, 10k stars), 350x400x160 (n=60, 50k stars), 700x800x320 (n=120,
-100k stars) and 700x1600x640 (n=120, 200k stars), r x mu x nu -- evaluated in
-one persistent launch replayed from a HIP graph, kernel variant chosen by the
-autotuner.  ``value`` is evaluations per second of that path; beside it the
-same 64 vectors evaluated one at a time in plain FP64, the full-size FP64
-evaluation (700x1600x640 plus a 700x400x160 cut, n=120, 3 streams, 200k
-stars), a short Poisson serving run, and the CPU baseline.
+evaluations against plain FP64, at per-star |d ln P| <= 1e-7.  A step is one
+batch of 64 parameter vectors (4 streams each, 16 per wedge) over 4 resident
+wedges of uneven size -- 70x160x64 (n=30, 10k stars), 350x400x160 (n=60, 50k
+stars), 700x800x320 (n=120, 100k stars) and 700x1600x640 (n=120, 200k stars),
+r x mu x nu -- evaluated in one persistent launch replayed from a HIP graph,
+kernel variant chosen by the autotuner.  ``value`` is evaluations per second
+of that path with the resampled quadrature (MW_PREC_RESAMPLED); its accuracy
+against the FP64 path is measured in the same run and reported.  Beside it:
+the same batch with the exact quadrature (MW_PREC_MIXED), the same 64 vectors
+one at a time in plain FP64, the full-size FP64 evaluation (700x1600x640 plus
+a 700x400x160 cut, n=120, 3 streams, 200k stars), a short Poisson serving run,
+and the CPU baseline.
 
 ``--workload full`` times single full-size evaluations instead (``--precision``
-fp64 or mixed).  Inputs (tables, stars) are resident in HBM before every
+fp64, mixed or resampled).  Inputs (tables, stars) are resident in HBM before every
 timed region; each step uploads its parameter vectors and reads back its
 results.  Prints one JSON line (the last line of stdout).  --gpus above 1
 exits: this build runs on one GPU.
@@ -115,7 +118,7 @@ def full_size_fp64(steps: int, warmup: int, precision=sep.PREC_FP64):
     vecs = synth.parameter_vectors(1, p, 4)
     for i in range(warmup):
         ctx.eval(vecs[i % 4], precision=precision)
-    lat, kern, star = [], [], []
+    lat, kern, star, points = [], [], [], []
     t0 = time.perf_counter()
     for i in range(steps):
         s0 = time.perf_counter()
@@ -124,15 +127,19 @@ def full_size_fp64(steps: int, warmup: int, precision=sep.PREC_FP64):
         tm = ctx.timing()
         kern.append(tm.integral_ms)
         star.append(tm.star_ms)
+        points.append(tm.integral_points)
     el = time.perf_counter() - t0
     ctx.close()
-    flops = workmodel.integral_flops(p.regions, p.convolve, synth.FULL_SIZE_STREAMS)
+    # Integrand evaluations of the integral pass: cells x n for the exact
+    # quadrature (workmodel.integral_points), grid nodes when resampled.
+    flops = float(np.mean(points)) * workmodel.flops_per_point(synth.FULL_SIZE_STREAMS)
     k_ms = float(np.mean(kern))
     return p, stars, {
         "value": steps / el, "unit": "evals/s", "ms_per_eval": 1e3 * el / steps,
         "integral_kernel_ms": round(k_ms, 3), "star_kernel_ms": round(float(np.mean(star)), 4),
         "achieved_tflops": round(flops / (k_ms * 1e-3) / 1e12, 3),
         "frac_fp64_peak": round(flops / (k_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
+        "integrand_evaluations": int(np.mean(points)), "flop_eq": flops,
         "resident_setup_s": round(setup, 3),
         "workload": "700x1600x640 + 700x400x160 cut (r x mu x nu), n=120, 3 streams, 200k stars",
     }
@@ -141,7 +148,7 @@ def full_size_fp64(steps: int, warmup: int, precision=sep.PREC_FP64):
 def serving_run(n_requests: int, rate: float, variant: int, bpc: int):
     wedges = serving.make_wedges(8, seed=4000)
     holders = [(ParamsHolder(p), StarsHolder(s)) for p, s, _ in wedges]
-    reqs = serving.make_requests(wedges, n_requests, seed=5000)
+    reqs = serving.make_requests(wedges, n_requests, seed=5000, fast_precision=sep.PREC_RESAMPLED)
     arr = serving.poisson_arrivals(len(reqs), rate, seed=6000)
     out = {}
     with sep.Server(max_inflight=3, max_batch=32, variant=variant, blocks_per_cu=bpc) as srv:
@@ -151,7 +158,7 @@ def serving_run(n_requests: int, rate: float, variant: int, bpc: int):
     with sep.Server(max_inflight=1, max_batch=1) as srv:
         ids = [srv.add_wedge(ph, sh) for ph, sh in holders]
         plain = serving.run(srv, ids, reqs, arr, force_precision=sep.PREC_FP64)
-    for k, v in (("continuous_batching_mixed", served), ("one_at_a_time_fp64", plain)):
+    for k, v in (("continuous_batching_resampled_mixed", served), ("one_at_a_time_fp64", plain)):
         out[k] = {"throughput_req_per_s": round(v["throughput"], 3),
                   "p50_ms": round(v["p50_ms"], 2), "p99_ms": round(v["p99_ms"], 2)}
     out["requests"] = n_requests
@@ -164,66 +171,100 @@ def serving_run(n_requests: int, rate: float, variant: int, bpc: int):
 
 # --------------------------------------------------------------------------
 
-def bench_batch(args):
-    t0 = time.perf_counter()
-    groups, vecs, plist = build_batch()
-    log(f"batch workload resident in {time.perf_counter() - t0:.2f} s")
-    tune_t0 = time.perf_counter()
-    choice = autotune.tune(groups, vecs, sep.PREC_MIXED, proxy_vectors=1, reps=1,
-                           blocks_per_cu=(2, 4), use_cache=False, log=log)
-    tune_s = time.perf_counter() - tune_t0
-    log(f"autotuner: variant {choice['variant']} blocks/CU {choice['blocks_per_cu']} ({tune_s:.1f} s)")
-    pairs = [(c, n) for c, n, _ in groups]
+def timed_batch(pairs, vecs, precision, choice, steps, warmup):
+    """Eager run (kernel events around the persistent launch), then `steps`
+    graph replays after `warmup`; returns evals/s, per-batch latencies,
+    elapsed seconds and the eager timing."""
     n_vec = sum(n for _, n in pairs)
-
-    # Roofline: one eager run times the persistent integral launch alone.
-    with sep.Batch(pairs, precision=sep.PREC_MIXED, graph=False) as eager:
+    with sep.Batch(pairs, precision=precision, graph=False) as eager:
         eager.set_variant(choice["variant"], choice["blocks_per_cu"])
         eager.run(vecs)
         eager.run(vecs)
         tm = eager.timing()
-    int_ms = tm.integral_ms
-    flops = sum(workmodel.integral_flops(p.regions, p.convolve, BATCH_STREAMS) * BATCH_VECTORS_PER_WEDGE
-                for p, _ in plist)
-    bytes_alg = sum(workmodel.integral_bytes(p.regions, p.convolve) * BATCH_VECTORS_PER_WEDGE
-                    for p, _ in plist)
-    log(f"eager batch: persistent integral launch {int_ms:.1f} ms of {tm.total_ms:.1f} ms")
-
-    with sep.Batch(pairs, precision=sep.PREC_MIXED, graph=True) as b:
+    with sep.Batch(pairs, precision=precision, graph=True) as b:
         b.set_variant(choice["variant"], choice["blocks_per_cu"])
-        for i in range(args.warmup):
+        for _ in range(warmup):
             b.run(vecs)
         sep.synchronize()
         lat = []
         t0 = time.perf_counter()
-        for i in range(args.steps):
+        for i in range(steps):
             s0 = time.perf_counter()
-            res = b.run(vecs)
+            b.run(vecs)
             lat.append(time.perf_counter() - s0)
-            log(f"step {i}: batch of {n_vec} in {lat[-1] * 1e3:.1f} ms (device {b.timing().total_ms:.1f} ms)")
+            log(f"  step {i}: batch of {n_vec} in {lat[-1] * 1e3:.1f} ms")
         sep.synchronize()
         elapsed = time.perf_counter() - t0
-    value = args.steps * n_vec / elapsed
+    return steps * n_vec / elapsed, lat, elapsed, tm
+
+
+def accuracy_vs_fp64(groups, vecs, precision, per_wedge=2):
+    """Largest per-star |d ln P| and integral relative errors of `precision`
+    against the GPU FP64 path (itself within 1e-12 of the FP64 oracle), over
+    the first vectors of every wedge."""
+    worst = {"per_star_dlnP": 0.0, "integral_rel": 0.0, "dlnL": 0.0}
+    for (ctx, _, _), v in zip(groups, vecs):
+        rows = np.asarray(v[:per_wedge])
+        ref = ctx.eval(rows, precision=sep.PREC_FP64, per_star=True)
+        got = ctx.eval(rows, precision=precision, per_star=True)
+        for a, b in zip(got, ref):
+            worst["per_star_dlnP"] = max(worst["per_star_dlnP"],
+                                         float(np.log(10) * np.max(np.abs(a.log10_p - b.log10_p))))
+            ints_a = np.concatenate([[a.background_integral], a.stream_integrals])
+            ints_b = np.concatenate([[b.background_integral], b.stream_integrals])
+            worst["integral_rel"] = max(worst["integral_rel"],
+                                        float(np.max(np.abs(ints_a - ints_b) / np.abs(ints_b))))
+            worst["dlnL"] = max(worst["dlnL"], float(np.log(10) * abs(a.likelihood - b.likelihood)))
+    return {k: float(f"{v:.3e}") for k, v in worst.items()}
+
+
+def bench_batch(args):
+    t0 = time.perf_counter()
+    groups, vecs, plist = build_batch()
+    log(f"batch workload resident in {time.perf_counter() - t0:.2f} s")
+    pairs = [(c, n) for c, n, _ in groups]
+    n_vec = sum(n for _, n in pairs)
+
+    tune_t0 = time.perf_counter()
+    choice = autotune.tune(groups, vecs, sep.PREC_RESAMPLED, proxy_vectors=1, reps=1,
+                           blocks_per_cu=(2, 4), use_cache=False, log=log)
+    tune_s = time.perf_counter() - tune_t0
+    log(f"autotuner (resampled): variant {choice['variant']} blocks/CU {choice['blocks_per_cu']} "
+        f"({tune_s:.1f} s)")
+    log("batched mixed precision, resampled quadrature")
+    value, lat, elapsed, tm = timed_batch(pairs, vecs, sep.PREC_RESAMPLED, choice, args.steps, args.warmup)
+    int_ms, points = tm.integral_ms, tm.integral_points
+    flops = float(points) * workmodel.flops_per_point(BATCH_STREAMS)
+    log(f"  eager: persistent integral launch {int_ms:.2f} ms of {tm.total_ms:.2f} ms, "
+        f"{points:.3e} integrand evaluations")
+
+    tune2 = autotune.tune(groups, vecs, sep.PREC_MIXED, proxy_vectors=1, reps=1,
+                          blocks_per_cu=(2, 4), use_cache=False, log=log)
+    log("batched mixed precision, exact quadrature")
+    value_ex, lat_ex, elapsed_ex, tm_ex = timed_batch(pairs, vecs, sep.PREC_MIXED, tune2,
+                                                      max(1, min(args.steps, 2)), 1)
+    flops_ex = sum(workmodel.integral_flops(p.regions, p.convolve, BATCH_STREAMS) * BATCH_VECTORS_PER_WEDGE
+                   for p, _ in plist)
+    bytes_alg = sum(workmodel.integral_bytes(p.regions, p.convolve) * BATCH_VECTORS_PER_WEDGE
+                    for p, _ in plist)
 
     # Plain FP64, one vector at a time, same vectors.
+    log("plain FP64, one at a time")
     t0 = time.perf_counter()
     for (ctx, n, _), v in zip(groups, vecs):
         for row in v:
             ctx.eval(row, precision=sep.PREC_FP64)
     sep.synchronize()
-    plain_s = time.perf_counter() - t0
-    plain = n_vec / plain_s
-    log(f"plain FP64 one at a time: {plain:.3f} evals/s")
+    plain = n_vec / (time.perf_counter() - t0)
+    log(f"  {plain:.3f} evals/s")
 
-    # Mixed precision one at a time (batching's own share of the gain).
-    t0 = time.perf_counter()
-    for (ctx, n, _), v in zip(groups, vecs):
-        for row in v[:4]:
-            ctx.eval(row, precision=sep.PREC_MIXED)
-    mixed_single = 4 * len(groups) / (time.perf_counter() - t0)
+    log("accuracy against FP64")
+    acc = accuracy_vs_fp64(groups, vecs, sep.PREC_RESAMPLED)
+    acc_ex = accuracy_vs_fp64(groups, vecs, sep.PREC_MIXED, per_wedge=1)
+    log(f"  resampled {acc}  exact-quadrature mixed {acc_ex}")
 
     out = {
-        "metric": "likelihood evals/s, batched mixed precision vs plain FP64",
+        "metric": "likelihood evals/s, batched mixed precision vs plain FP64 @ per-star |d lnP| <= 1e-7",
         "value": round(value, 4), "unit": "evals/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(1e3 * elapsed / args.steps, 3),
@@ -236,29 +277,46 @@ def bench_batch(args):
                          "700x1600x640 n=120 200k (r x mu x nu); one persistent launch "
                          "replayed from a HIP graph"),
             "batch": n_vec, "precision": "mixed",
+            "quadrature": ("resampled: each region's r-step x convolution weights folded by 16-node "
+                           "Lagrange interpolation onto a magnitude grid sized to the model's narrowest "
+                           "feature (MW_PREC_RESAMPLED); star pass exact"),
             "kernel_variant": choice["variant"], "blocks_per_cu": choice["blocks_per_cu"],
             "parallelism": "single GPU",
         },
+        "accuracy_vs_fp64": dict(acc, bar_per_star_dlnP=1e-7,
+                                 what="worst over 2 vectors per wedge, against the GPU FP64 path"),
         "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 2),
                        "p99": round(1e3 * float(np.percentile(lat, 99)), 2),
                        "what": "one batch of 64 evaluations"},
         "plain_fp64": {"value": round(plain, 4), "unit": "evals/s",
                        "ratio_batched_mixed_over_plain_fp64": round(value / plain, 3),
-                       "what": "the same 64 vectors, one evaluation at a time, FP64"},
-        "single_mixed": {"value": round(mixed_single, 4), "unit": "evals/s",
-                         "what": "16 of the vectors one at a time, mixed precision"},
+                       "what": "the same 64 vectors, one evaluation at a time, FP64, exact quadrature"},
+        "batched_mixed_exact_quadrature": {
+            "value": round(value_ex, 4), "unit": "evals/s",
+            "ratio_over_plain_fp64": round(value_ex / plain, 3),
+            "accuracy_vs_fp64": acc_ex,
+            "p50_ms": round(1e3 * float(np.percentile(lat_ex, 50)), 2),
+            "kernel_variant": tune2["variant"], "blocks_per_cu": tune2["blocks_per_cu"],
+            "roofline": {"bound": "valu", "kernel": "integral_persistent (mixed, exact quadrature)",
+                         "achieved": round(flops_ex / (tm_ex.integral_ms * 1e-3) / 1e12, 3),
+                         "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(flops_ex / (tm_ex.integral_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
+                         "traffic": measured_traffic("integral_persistent_mixed_batch64"),
+                         "algorithmic_flop_eq_per_launch": flops_ex,
+                         "algorithmic_bytes_per_launch": bytes_alg,
+                         "avg_launch_ms": round(tm_ex.integral_ms, 3)},
+        },
         "roofline": {
-            "bound": "valu", "kernel": "integral_persistent (mixed)",
+            "bound": "valu", "kernel": "integral_persistent (mixed, resampled quadrature)",
             "achieved": round(flops / (int_ms * 1e-3) / 1e12, 3), "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(flops / (int_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
-            "traffic": measured_traffic("integral_persistent_mixed_batch64"),
+            "traffic": measured_traffic("integral_persistent_resampled_batch64"),
+            "integrand_evaluations_per_launch": int(points),
             "algorithmic_flop_eq_per_launch": flops,
-            "algorithmic_bytes_per_launch": bytes_alg,
             "avg_launch_ms": round(int_ms, 3),
         },
-        "tuning": {"seconds": round(tune_s, 2),
-                   "table": choice["table"]},
+        "tuning": {"seconds": round(tune_s, 2), "table": choice["table"]},
     }
     for ctx, *_ in groups:
         ctx.close()
@@ -290,9 +348,9 @@ def bench_batch(args):
 
 
 def bench_full(args):
-    prec = sep.PREC_MIXED if args.precision == "mixed" else sep.PREC_FP64
+    prec = {"fp64": sep.PREC_FP64, "mixed": sep.PREC_MIXED, "resampled": sep.PREC_RESAMPLED}[args.precision]
     p, stars, full = full_size_fp64(args.steps, args.warmup, prec)
-    flops = workmodel.integral_flops(p.regions, p.convolve, synth.FULL_SIZE_STREAMS)
+    flops = full["flop_eq"]
     out = {
         "metric": "likelihood evals/s (separation integral + star pass)",
         "value": round(full["value"], 4), "unit": "evals/s", "n_gpus": 1,
@@ -301,6 +359,7 @@ def bench_full(args):
         "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if prec == sep.PREC_FP64 else "f64+f32",
         "data": "synthetic (seeded wedge geometry, parameters and stars)",
+        "integrand_evaluations": full["integrand_evaluations"],
         "config": {"workload": "full: " + full["workload"], "precision": args.precision,
                    "batch": 1, "parallelism": "single GPU"},
         "star_kernel_ms": full["star_kernel_ms"],
@@ -332,7 +391,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["batch", "full"], default="batch")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+    ap.add_argument("--precision", choices=["fp64", "mixed", "resampled"], default="fp64",
                     help="precision of --workload full")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-full", action="store_true", help="skip the full-size FP64 line")

@@ -122,6 +122,9 @@ typedef struct {
     double star_ms;                    /* star kernel(s) */
     double total_ms;                   /* first to last kernel, including reductions */
     long integral_blocks, star_blocks; /* workgroups launched */
+    long integral_points;              /* integrand evaluations of the integral pass: cells x
+                                          convolve points, or sights x grid nodes where the
+                                          quadrature is resampled */
 } mw_timing;
 
 /* Flags for mw_ctx_create */

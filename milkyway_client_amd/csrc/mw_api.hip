@@ -149,6 +149,7 @@ int run_plan(mw_ctx* c, Plan& P)
     struct Group { int bg; int first_item, n_items, first_block, n_blocks; };
     std::vector<Group> igroups;
     int total_iblocks = 0;
+    long points = 0;
     if (P.integrals && n_pr > 0) {
         if ((rc = c->ritems.reserve(n_pr)) || (rc = c->h_ritems.reserve(n_pr)) ||
             (rc = c->rred.reserve(n_pr)) || (rc = c->h_rred.reserve(n_pr)) ||
@@ -184,6 +185,9 @@ int run_plan(mw_ctx* c, Plan& P)
                 // index space is columns x grid rows.
                 L.calc_begin = 0;
                 L.calc_end = rd.n_cols * (long)tr.r_steps;
+                points += rd.n_cols * tr.nodes;
+            } else {
+                points += (pr.end - pr.begin) * (long)rd.n_conv;
             }
             L.slot = pr.slot;
             L.out_index = order[oi];
@@ -325,6 +329,7 @@ int run_plan(mw_ctx* c, Plan& P)
         c->timing.total_ms = t;
     }
     c->timing.integral_blocks = total_iblocks;
+    c->timing.integral_points = points;
     c->timing.star_blocks = total_sblocks;
     return MW_OK;
 }

@@ -58,6 +58,7 @@ public:
     size_t scratch_bytes() const;
     hipStream_t stream() const { return stream_; }
     long integral_blocks() const { return total_iblocks_; }
+    long integral_points() const { return points_; }
     long star_blocks() const { return total_sblocks_; }
 
     static size_t scratch_bytes_for(const std::vector<Slot>& slots);
@@ -73,6 +74,7 @@ private:
     bool mixed_ = false, resample_ = false;
     int variant_ = 0, blocks_per_cu_ = 2;
     std::vector<int> levels_;        // quadrature level per (slot, region), slot-major
+    long points_ = 0;                // integrand evaluations of the planned integral pass
     std::vector<int> slot_bg_, slot_n_regions_, slot_first_row_, slot_S_;
     std::vector<Group> igroups_, sgroups_;
     int n_ritems_ = 0, total_iblocks_ = 0, total_sblocks_ = 0;

@@ -77,6 +77,7 @@ int BatchRunner::plan(const std::vector<Slot>& slots, int precision)
     mixed_ = precision != MW_PREC_FP64;
     resample_ = precision == MW_PREC_RESAMPLED;
     levels_.clear();
+    points_ = 0;
     slot_bg_.assign(n, 0);
     slot_n_regions_.assign(n, 0);
     slot_first_row_.assign(n, 0);
@@ -104,6 +105,7 @@ int BatchRunner::plan(const std::vector<Slot>& slots, int precision)
             int rc0 = region_tables(*rd, level, &tr);
             if (rc0) return rc0;
             levels_.push_back(level);
+            points_ += rd->n_cols * tr.nodes;
             RegionLaunch L;
             std::memset(&L, 0, sizeof(L));
             L.rq = tr.rq;
@@ -449,6 +451,7 @@ int mw_batch_run(mw_batch* b, const double* params, mw_result* results)
     b->timing.integral_ms = (b->flags & MW_BATCH_GRAPH) ? b->timing.total_ms : b->runner.integral_ms();
     b->timing.star_ms = 0;
     b->timing.integral_blocks = b->runner.integral_blocks();
+    b->timing.integral_points = b->runner.integral_points();
     b->timing.star_blocks = b->runner.star_blocks();
     return MW_OK;
 }

@@ -276,7 +276,8 @@ struct TableRef {
     const double* rq;
     const RStep* rstep;
     int r_steps, n_conv;
-    int level;   // -1 for the exact tables
+    int level;         // -1 for the exact tables
+    long nodes;        // integrand evaluations per sight: r_steps x n_conv, or the grid's nodes
 };
 
 // Resampled level to use for region rd under the model k, or -1 for the

@@ -54,7 +54,8 @@ int select_level(const RegionDev& rd, const EvalConst& k, bool resample)
 int region_tables(RegionDev& rd, int level, TableRef* out)
 {
     if (level < 0) {
-        *out = TableRef{ rd.rq.p, rd.rstep.p, rd.geom.r_steps, rd.n_conv, -1 };
+        *out = TableRef{ rd.rq.p, rd.rstep.p, rd.geom.r_steps, rd.n_conv, -1,
+                         (long)rd.geom.r_steps * rd.n_conv };
         return MW_OK;
     }
     if (level >= kQuadLevels) return fail(MW_E_INVALID, "no such quadrature level");
@@ -74,7 +75,7 @@ int region_tables(RegionDev& rd, int level, TableRef* out)
         rd.levels[level] = std::move(q);
     }
     const ResampledDev& q = *rd.levels[level];
-    *out = TableRef{ q.rq.p, q.rstep.p, q.r_steps, q.n_conv, level };
+    *out = TableRef{ q.rq.p, q.rstep.p, q.r_steps, q.n_conv, level, q.nodes };
     return MW_OK;
 }
 

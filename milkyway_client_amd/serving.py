@@ -40,14 +40,15 @@ def make_wedges(n_wedges: int = 8, seed: int = 4000, shapes=None, max_streams: i
     return out
 
 
-def make_requests(wedges, n: int, seed: int, fp64_fraction: float = 0.1) -> List[Request]:
+def make_requests(wedges, n: int, seed: int, fp64_fraction: float = 0.1,
+                  fast_precision: int = sep.PREC_MIXED) -> List[Request]:
     rng = np.random.Generator(np.random.PCG64(seed))
     reqs = []
     for j in range(n):
         w = int(rng.integers(0, len(wedges)))
         p = wedges[w][0]
         vec = synth.parameter_vectors(seed + j, p, 1)[0]
-        prec = sep.PREC_FP64 if rng.random() < fp64_fraction else sep.PREC_MIXED
+        prec = sep.PREC_FP64 if rng.random() < fp64_fraction else fast_precision
         reqs.append(Request(w, vec, prec))
     return reqs
 
