@@ -225,28 +225,28 @@ def bench_batch(args):
     pairs = [(c, n) for c, n, _ in groups]
     n_vec = sum(n for _, n in pairs)
 
+    head_prec = sep.PREC_RESAMPLED if args.batch_precision == "resampled" else sep.PREC_MIXED
+    other_prec = sep.PREC_MIXED if head_prec == sep.PREC_RESAMPLED else sep.PREC_RESAMPLED
     tune_t0 = time.perf_counter()
-    choice = autotune.tune(groups, vecs, sep.PREC_RESAMPLED, proxy_vectors=1, reps=1,
+    choice = autotune.tune(groups, vecs, head_prec, proxy_vectors=1, reps=1,
                            blocks_per_cu=(2, 4), use_cache=False, log=log)
     tune_s = time.perf_counter() - tune_t0
-    log(f"autotuner (resampled): variant {choice['variant']} blocks/CU {choice['blocks_per_cu']} "
-        f"({tune_s:.1f} s)")
-    log("batched mixed precision, resampled quadrature")
-    value, lat, elapsed, tm = timed_batch(pairs, vecs, sep.PREC_RESAMPLED, choice, args.steps, args.warmup)
+    log(f"autotuner ({args.batch_precision}): variant {choice['variant']} blocks/CU "
+        f"{choice['blocks_per_cu']} ({tune_s:.1f} s)")
+    log(f"batched mixed precision, headline ({args.batch_precision})")
+    value, lat, elapsed, tm = timed_batch(pairs, vecs, head_prec, choice, args.steps, args.warmup)
     int_ms, points = tm.integral_ms, tm.integral_points
     flops = float(points) * workmodel.flops_per_point(BATCH_STREAMS)
     log(f"  eager: persistent integral launch {int_ms:.2f} ms of {tm.total_ms:.2f} ms, "
         f"{points:.3e} integrand evaluations")
 
-    tune2 = autotune.tune(groups, vecs, sep.PREC_MIXED, proxy_vectors=1, reps=1,
-                          blocks_per_cu=(2, 4), use_cache=False, log=log)
-    log("batched mixed precision, exact quadrature")
-    value_ex, lat_ex, elapsed_ex, tm_ex = timed_batch(pairs, vecs, sep.PREC_MIXED, tune2,
-                                                      max(1, min(args.steps, 2)), 1)
-    flops_ex = sum(workmodel.integral_flops(p.regions, p.convolve, BATCH_STREAMS) * BATCH_VECTORS_PER_WEDGE
-                   for p, _ in plist)
-    bytes_alg = sum(workmodel.integral_bytes(p.regions, p.convolve) * BATCH_VECTORS_PER_WEDGE
-                    for p, _ in plist)
+    second = None
+    if not args.no_secondary:
+        tune2 = autotune.tune(groups, vecs, other_prec, proxy_vectors=1, reps=1,
+                              blocks_per_cu=(2, 4), use_cache=False, log=log)
+        log("batched mixed precision, the other quadrature")
+        v2, lat2, _, tm2 = timed_batch(pairs, vecs, other_prec, tune2, max(1, min(args.steps, 2)), 1)
+        second = (v2, lat2, tm2, tune2)
 
     # Plain FP64, one vector at a time, same vectors.
     log("plain FP64, one at a time")
@@ -259,10 +259,14 @@ def bench_batch(args):
     log(f"  {plain:.3f} evals/s")
 
     log("accuracy against FP64")
-    acc = accuracy_vs_fp64(groups, vecs, sep.PREC_RESAMPLED)
-    acc_ex = accuracy_vs_fp64(groups, vecs, sep.PREC_MIXED, per_wedge=1)
-    log(f"  resampled {acc}  exact-quadrature mixed {acc_ex}")
+    acc = accuracy_vs_fp64(groups, vecs, head_prec)
+    log(f"  {args.batch_precision}: {acc}")
 
+    quad = "resampled" if head_prec == sep.PREC_RESAMPLED else "exact"
+    quad_desc = ("resampled: each region's r-step x convolution weights folded by 16-node Lagrange "
+                 "interpolation onto a magnitude grid sized to the model's narrowest feature "
+                 "(MW_PREC_RESAMPLED); star pass exact" if quad == "resampled" else
+                 "exact: the reference's r-step x convolution points (MW_PREC_MIXED)")
     out = {
         "metric": "likelihood evals/s, batched mixed precision vs plain FP64 @ per-star |d lnP| <= 1e-7",
         "value": round(value, 4), "unit": "evals/s", "n_gpus": 1,
@@ -277,9 +281,7 @@ def bench_batch(args):
                          "700x1600x640 n=120 200k (r x mu x nu); one persistent launch "
                          "replayed from a HIP graph"),
             "batch": n_vec, "precision": "mixed",
-            "quadrature": ("resampled: each region's r-step x convolution weights folded by 16-node "
-                           "Lagrange interpolation onto a magnitude grid sized to the model's narrowest "
-                           "feature (MW_PREC_RESAMPLED); star pass exact"),
+            "quadrature": quad_desc,
             "kernel_variant": choice["variant"], "blocks_per_cu": choice["blocks_per_cu"],
             "parallelism": "single GPU",
         },
@@ -291,33 +293,38 @@ def bench_batch(args):
         "plain_fp64": {"value": round(plain, 4), "unit": "evals/s",
                        "ratio_batched_mixed_over_plain_fp64": round(value / plain, 3),
                        "what": "the same 64 vectors, one evaluation at a time, FP64, exact quadrature"},
-        "batched_mixed_exact_quadrature": {
-            "value": round(value_ex, 4), "unit": "evals/s",
-            "ratio_over_plain_fp64": round(value_ex / plain, 3),
-            "accuracy_vs_fp64": acc_ex,
-            "p50_ms": round(1e3 * float(np.percentile(lat_ex, 50)), 2),
-            "kernel_variant": tune2["variant"], "blocks_per_cu": tune2["blocks_per_cu"],
-            "roofline": {"bound": "valu", "kernel": "integral_persistent (mixed, exact quadrature)",
-                         "achieved": round(flops_ex / (tm_ex.integral_ms * 1e-3) / 1e12, 3),
-                         "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                         "frac": round(flops_ex / (tm_ex.integral_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
-                         "traffic": measured_traffic("integral_persistent_mixed_batch64"),
-                         "algorithmic_flop_eq_per_launch": flops_ex,
-                         "algorithmic_bytes_per_launch": bytes_alg,
-                         "avg_launch_ms": round(tm_ex.integral_ms, 3)},
-        },
         "roofline": {
-            "bound": "valu", "kernel": "integral_persistent (mixed, resampled quadrature)",
+            "bound": "valu", "kernel": f"integral_persistent (mixed, {quad} quadrature)",
             "achieved": round(flops / (int_ms * 1e-3) / 1e12, 3), "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(flops / (int_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
-            "traffic": measured_traffic("integral_persistent_resampled_batch64"),
+            "traffic": measured_traffic(f"integral_persistent_{args.batch_precision}_batch64"),
             "integrand_evaluations_per_launch": int(points),
             "algorithmic_flop_eq_per_launch": flops,
             "avg_launch_ms": round(int_ms, 3),
         },
         "tuning": {"seconds": round(tune_s, 2), "table": choice["table"]},
     }
+    if second:
+        v2, lat2, tm2, tune2 = second
+        q2 = "exact" if quad == "resampled" else "resampled"
+        pts2 = float(tm2.integral_points)
+        f2 = pts2 * workmodel.flops_per_point(BATCH_STREAMS)
+        key2 = "mixed" if q2 == "exact" else "resampled"
+        out[f"batched_mixed_{q2}_quadrature"] = {
+            "value": round(v2, 4), "unit": "evals/s", "ratio_over_plain_fp64": round(v2 / plain, 3),
+            "accuracy_vs_fp64": accuracy_vs_fp64(groups, vecs, other_prec, per_wedge=1),
+            "p50_ms": round(1e3 * float(np.percentile(lat2, 50)), 2),
+            "kernel_variant": tune2["variant"], "blocks_per_cu": tune2["blocks_per_cu"],
+            "roofline": {"bound": "valu", "kernel": f"integral_persistent (mixed, {q2} quadrature)",
+                         "achieved": round(f2 / (tm2.integral_ms * 1e-3) / 1e12, 3),
+                         "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(f2 / (tm2.integral_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
+                         "traffic": measured_traffic(f"integral_persistent_{key2}_batch64"),
+                         "integrand_evaluations_per_launch": int(pts2),
+                         "algorithmic_flop_eq_per_launch": f2,
+                         "avg_launch_ms": round(tm2.integral_ms, 3)},
+        }
     for ctx, *_ in groups:
         ctx.close()
 
@@ -396,6 +403,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-full", action="store_true", help="skip the full-size FP64 line")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving run")
+    ap.add_argument("--batch-precision", choices=["resampled", "mixed"], default="resampled",
+                    help="precision of the headline batch (the other one runs beside it)")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="skip the second batch precision (for profiling one path)")
     ap.add_argument("--serving-requests", type=int, default=40)
     ap.add_argument("--serving-rate", type=float, default=10.0)
     args = ap.parse_args()
