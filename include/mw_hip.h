@@ -45,13 +45,14 @@ extern "C" {
  * not the drop-in entry points).  The reference's integral sums, per sight,
  * r_steps x convolve weighted densities at distances that are the same for
  * every sight.  Here those weights are folded, by 16-node Lagrange
- * interpolation, onto a uniform grid in magnitude whose spacing follows the
- * narrowest feature the model can have (the thinnest stream at the farthest
- * distance, the closest approach to the Galactic centre), and the sum is
- * taken over the grid: the same quadrature to about 1e-10 per sight, from up
- * to 40-80x fewer density evaluations on large wedges.  Regions where it
- * would not pay, and background modes other than the Hernquist profile,
- * use the exact tables.  The star pass is the exact one. */
+ * interpolation, onto a uniform grid in distance or in magnitude (whichever
+ * needs fewer nodes) whose spacing follows the narrowest feature the model
+ * can have: the thinnest stream, and the sights' closest approach to the
+ * Galactic centre.  The sum is taken over the grid: the same quadrature to
+ * about 1e-10 per sight, from up to hundreds of times fewer density
+ * evaluations on large wedges.  Regions where it would not pay, and
+ * background modes other than the Hernquist profile, use the exact tables.
+ * The star pass is the exact one. */
 #define MW_PREC_RESAMPLED 2
 
 /* The weight folding behind MW_PREC_RESAMPLED, on the host, for any point

@@ -96,15 +96,17 @@ int build_context(mw_ctx* c, const std::vector<RegionGeom>& geoms, int n_conv, i
         MW_HIP_CHECK(hipMemcpy(rd->col.p, t.col.data(), sizeof(double) * t.col.size(), hipMemcpyHostToDevice));
         // What the resampled quadrature needs: the exact tables on the host
         // and the region's extremes.
-        double gmin = INFINITY, gmax = -INFINITY;
+        double gmin = INFINITY, gmax = -INFINITY, rmin = INFINITY;
         for (size_t i = 0; i < t.rq.size(); i += 2) {
             const double r = t.rq[i];
             rd->r_max = std::max(rd->r_max, r);
+            rmin = std::min(rmin, r);
             const double gm = 5.0 * (std::log10(r * 1000.0) - 1.0) + kAbsM;
             gmin = std::min(gmin, gm);
             gmax = std::max(gmax, gm);
         }
         rd->g_span = gmax - gmin;
+        rd->r_span = rd->r_max - rmin;
         for (long col = 0; col < t.n_cols; col++) {
             const double ux = t.col[col];
             rd->min_sin_gc = std::min(rd->min_sin_gc, std::sqrt(std::max(0.0, 1.0 - ux * ux)));

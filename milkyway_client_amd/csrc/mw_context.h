@@ -243,8 +243,11 @@ inline void reset_state(EVALUATION_STATE* es)
     }
 }
 
-// Resampled quadrature tables of one region at one grid spacing.
-constexpr int kQuadLevels = 13;   // grid spacing 0.02 * 2^(-L/2) magnitudes, L < kQuadLevels
+// Resampled quadrature tables of one region at one grid spacing.  Table
+// index t < kQuadLevels: a grid in magnitude with spacing 0.02 * 2^(-t/2);
+// t = kQuadLevels + L: a grid in distance with spacing 2 * 2^(-L/2) kpc.
+constexpr int kQuadLevels = 13;
+constexpr int kQuadTables = 2 * kQuadLevels;
 struct ResampledDev {
     int r_steps = 0, n_conv = 0;
     long nodes = 0;
@@ -261,13 +264,13 @@ struct RegionDev {
     DevBuf<RStep> rstep;
     // Host copies of the exact r-step tables, and what the choice of a
     // resampled grid needs: the largest convolution distance (kpc), the
-    // magnitude span of the convolution points, and the smallest
+    // distance and magnitude spans of the convolution points, and the smallest
     // sqrt(1 - u_x^2) over the region's sights (their closest approach to
     // the Galactic centre is 8.5 kpc times that for q = 1).
     std::vector<double> h_rq;
     std::vector<RStep> h_rstep;
-    double r_max = 0.0, g_span = 0.0, min_sin_gc = 1.0;
-    std::unique_ptr<ResampledDev> levels[kQuadLevels];
+    double r_max = 0.0, r_span = 0.0, g_span = 0.0, min_sin_gc = 1.0;
+    std::unique_ptr<ResampledDev> levels[kQuadTables];
     std::mutex level_mu;
 };
 
@@ -280,9 +283,9 @@ struct TableRef {
     long nodes;        // integrand evaluations per sight: r_steps x n_conv, or the grid's nodes
 };
 
-// Resampled level to use for region rd under the model k, or -1 for the
-// exact tables (resampling off, a background mode it does not cover, or no
-// gain).  Pure: builds nothing.
+// Resampled table index to use for region rd under the model k, or -1 for
+// the exact tables (resampling off, a background mode it does not cover, or
+// no gain).  Pure: builds nothing.
 int select_level(const RegionDev& rd, const EvalConst& k, bool resample);
 // The tables of that level, built and uploaded on first use.
 int region_tables(RegionDev& rd, int level, TableRef* out);

@@ -131,23 +131,24 @@ bool separation_frame(int wedge, int sgr, double normal[3], double rot[3][3]);
 // Resampled quadrature.  A region's integral per column is
 // sum_p w_p f(r_p) over its r-step x convolution points p, with weights w_p
 // (irv * reff * qw) and distances r_p that are the same for every column.
-// Folding degree-D Lagrange interpolation on a uniform grid g_m in magnitude
-// g = 5 log10(1000 r) - 5 + 4.2 into the weights gives
-// K_m = sum_p w_p L_m(g_p), and the column sum becomes sum_m K_m f(r(g_m)):
-// the same quadrature up to the interpolation error of f between grid nodes,
-// from far fewer density evaluations.
+// Folding degree-D Lagrange interpolation on a uniform grid x_m into the
+// weights gives K_m = sum_p w_p L_m(x_p), and the column sum becomes
+// sum_m K_m f(r(x_m)): the same quadrature up to the interpolation error of f
+// between grid nodes, from far fewer density evaluations.  The grid variable
+// x is the distance r itself or the magnitude g = 5 log10(1000 r) - 5 + 4.2.
 constexpr int kResampleDegree = 15;   // 16-node stencils
-constexpr int kResampleRow = 128;     // grid nodes per pseudo r-step row
+constexpr int kResampleRow = 32;      // grid nodes per pseudo r-step row (padding < 32)
 
 // K over the grid g_lo + m delta, m < M; returns M (or -1 on bad input).
 long resample_quadrature(const double* g, const double* w, long n, double delta, int degree,
                          double* g_lo, std::vector<double>* K);
 // Tables in the layout of RegionTables (rq, rstep only) holding the
-// resampled quadrature of an exact region's rq / rstep: pseudo r-steps of
-// kResampleRow (r(g_m), K_m) pairs with irv = reff = 1, and the aux-profile
-// moments sum_k irv reff q_i in the first row.  Returns M.
+// resampled quadrature of an exact region's rq / rstep on a grid of spacing
+// delta in r (in_r) or in g: pseudo r-steps of kResampleRow (r(x_m), K_m)
+// pairs with irv = reff = 1, and the aux-profile moments sum_k irv reff q_i
+// in the first row.  Returns M.
 long build_resampled_tables(const std::vector<double>& rq, const std::vector<RStep>& rstep, int n_conv,
-                            double delta, RegionTables* out);
+                            double delta, bool in_r, RegionTables* out);
 
 // Star-pass constants shared by every star of a context: r_i / r_star and
 // W_i * coeff * exp(-dx_i^2 / (2 sigma^2)).

@@ -1,16 +1,22 @@
 // mw_quad.hip -- choice and device copies of the resampled quadrature tables.
 //
 // The grid spacing is chosen per region and parameter vector from the
-// narrowest feature the integrand can have along a sight, in magnitudes:
-//   stream s: a Gaussian of width >= sigma_s along the sight, i.e. at least
-//             (5 / ln 10) sigma_s / r_max in g;
+// narrowest feature the integrand can have along a sight:
+//   stream s: a Gaussian of width >= sigma_s along the sight (in distance),
+//             i.e. >= (5 / ln 10) sigma_s / r_max in magnitude;
 //   background: 1 / (rg (rg + r0)^3) peaks where the sight passes closest to
 //             the Galactic centre, at distance b >= 8.5 min sin(GC angle)
-//             min(1, 1/q) and r <= 8.5, a width of at least (5 / ln 10) b / 8.5.
-// Interpolating with 16-node stencils at spacing 0.3 w (streams) or 0.2 w
-// (background) keeps the per-sight quadrature error below 1e-10 relative
-// (measured on these profiles: 3e-13 for a 1 kpc stream at 100 kpc with
-// spacing 0.005, 1e-12 for a sight 0.14 kpc from the centre).
+//             min(1, 1/q) and r <= 8.5: a width of at least b in distance,
+//             (5 / ln 10) b / 8.5 in magnitude.
+// A grid uniform in distance resolves the streams with the same spacing at
+// every distance; a grid in magnitude is finer near the Sun, which is where a
+// sight passing close to the Galactic centre needs it.  Both are sized, the
+// one with fewer nodes is used.  With 16-node stencils at spacing 0.25 sigma
+// (distance) or 0.3 w (magnitude) for streams and 0.15 b / 0.2 w for the
+// background, the per-sight quadrature error stays near 1e-10 relative
+// (measured on a full-size point set: 1 kpc streams crossing out to 100 kpc,
+// 8e-10 at 0.3 sigma and 9e-13 at 0.2 sigma in distance, 3e-13 at 0.005 mag
+// in magnitude; a sight 0.14 kpc from the centre, 1e-12 at 0.005 mag).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -22,9 +28,29 @@ namespace mw {
 namespace {
 
 constexpr double kMagPerLn = 2.1714724095162588;   // 5 / ln 10
-constexpr double kDeltaMax = 0.02;
+constexpr double kDeltaMag = 0.02;                  // coarsest magnitude spacing
+constexpr double kDeltaKpc = 2.0;                   // coarsest distance spacing
 
-double level_delta(int L) { return kDeltaMax * std::pow(2.0, -0.5 * L); }
+// Spacing of table index t (see kQuadLevels).
+double table_delta(int t)
+{
+    return t < kQuadLevels ? kDeltaMag * std::pow(2.0, -0.5 * t)
+                           : kDeltaKpc * std::pow(2.0, -0.5 * (t - kQuadLevels));
+}
+
+// Finest level no coarser than delta, or kQuadLevels if none is.
+int level_for(double delta, double coarsest)
+{
+    if (!(delta > 0.0)) return kQuadLevels;
+    const double d = std::min(delta, coarsest);
+    return std::max(0, (int)std::ceil(-2.0 * std::log2(d / coarsest) - 1e-12));
+}
+
+long nodes_for(double span, double delta)
+{
+    const long n = (long)std::ceil(span / delta) + kResampleDegree + 4;
+    return (n + kResampleRow - 1) / kResampleRow * kResampleRow;   // row padding
+}
 
 }  // namespace
 
@@ -32,23 +58,34 @@ int select_level(const RegionDev& rd, const EvalConst& k, bool resample)
 {
     if (!resample || rd.h_rq.empty()) return -1;
     if (k.bg_mode != kBgHernquist && k.bg_mode != kBgHernquistAux) return -1;
-    double w = INFINITY;
+    double sigma = INFINITY;
     for (int s = 0; s < k.n_streams; s++) {
         if (k.on[s] == 0.0 || !(k.inv_2s2[s] > 0.0)) continue;
-        const double sigma = std::sqrt(0.5 / k.inv_2s2[s]);
-        w = std::min(w, 0.3 * kMagPerLn * sigma / rd.r_max);
+        sigma = std::min(sigma, std::sqrt(0.5 / k.inv_2s2[s]));
     }
     const double b = kSunR0 * rd.min_sin_gc * std::min(1.0, 1.0 / k.q);
-    w = std::min(w, 0.2 * kMagPerLn * b / kSunR0);
-    const double delta = std::min(w, kDeltaMax);
-    if (!(delta > 0.0)) return -1;
-    const int L = (int)std::ceil(-2.0 * std::log2(delta / kDeltaMax) - 1e-12);
-    if (L >= kQuadLevels) return -1;
-    // Grid nodes (with stencil margins and row padding) against exact points.
-    const long nodes = (long)std::ceil(rd.g_span / level_delta(L)) + kResampleDegree + 4;
-    const long padded = (nodes + kResampleRow - 1) / kResampleRow * kResampleRow;
-    if (padded * 5 >= (long)rd.geom.r_steps * rd.n_conv * 4) return -1;
-    return L;
+    // grid in magnitude
+    const double dg = std::min(0.3 * kMagPerLn * sigma / rd.r_max, 0.2 * kMagPerLn * b / kSunR0);
+    const int Lg = level_for(dg, kDeltaMag);
+    // grid in distance
+    const double dr = std::min(0.25 * sigma, 0.15 * b);
+    const int Lr = level_for(dr, kDeltaKpc);
+    long best = -1;
+    int t = -1;
+    if (Lg < kQuadLevels) {
+        best = nodes_for(rd.g_span, table_delta(Lg));
+        t = Lg;
+    }
+    if (Lr < kQuadLevels) {
+        const long n = nodes_for(rd.r_span, table_delta(kQuadLevels + Lr));
+        if (best < 0 || n < best) {
+            best = n;
+            t = kQuadLevels + Lr;
+        }
+    }
+    // Keep the exact tables unless the grid saves at least a fifth.
+    if (t < 0 || best * 5 >= (long)rd.geom.r_steps * rd.n_conv * 4) return -1;
+    return t;
 }
 
 int region_tables(RegionDev& rd, int level, TableRef* out)
@@ -58,11 +95,12 @@ int region_tables(RegionDev& rd, int level, TableRef* out)
                          (long)rd.geom.r_steps * rd.n_conv };
         return MW_OK;
     }
-    if (level >= kQuadLevels) return fail(MW_E_INVALID, "no such quadrature level");
+    if (level >= kQuadTables) return fail(MW_E_INVALID, "no such quadrature table");
     std::lock_guard<std::mutex> lock(rd.level_mu);
     if (!rd.levels[level]) {
         RegionTables t;
-        const long M = build_resampled_tables(rd.h_rq, rd.h_rstep, rd.n_conv, level_delta(level), &t);
+        const long M = build_resampled_tables(rd.h_rq, rd.h_rstep, rd.n_conv, table_delta(level),
+                                              level >= kQuadLevels, &t);
         if (M < 1) return fail(MW_E_INVALID, "resampled quadrature failed");
         auto q = std::make_unique<ResampledDev>();
         q->r_steps = t.r_steps;

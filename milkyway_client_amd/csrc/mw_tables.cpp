@@ -522,7 +522,7 @@ long resample_quadrature(const double* g, const double* w, long n, double delta,
 }
 
 long build_resampled_tables(const std::vector<double>& rq, const std::vector<RStep>& rstep, int n_conv,
-                            double delta, RegionTables* out)
+                            double delta, bool in_r, RegionTables* out)
 {
     const long R = (long)rstep.size();
     const long N = R * n_conv;
@@ -535,7 +535,7 @@ long build_resampled_tables(const std::vector<double>& rq, const std::vector<RSt
         Q2 += f * rstep[k].q2;
         for (long j = 0; j < n_conv; j++) {
             const double r = rq[2 * (k * n_conv + j)];
-            g[k * n_conv + j] = 5.0 * (std::log10(r * 1000.0) - 1.0) + kAbsM;
+            g[k * n_conv + j] = in_r ? r : 5.0 * (std::log10(r * 1000.0) - 1.0) + kAbsM;
             w[k * n_conv + j] = f * rq[2 * (k * n_conv + j) + 1];
         }
     }
@@ -548,10 +548,14 @@ long build_resampled_tables(const std::vector<double>& rq, const std::vector<RSt
     out->r_steps = (int)Rp;
     out->n_conv = row;
     out->rq.assign((size_t)2 * Rp * row, 0.0);
-    const double r_last = std::pow(10.0, (lo + (M - 1) * delta - kAbsM) / 5.0 + 1.0) / 1000.0;
+    // Node distances: the grid itself in r, or r(g) = 10^((g - 4.2) / 5 + 1) / 1000.
+    auto node_r = [&](long m) {
+        const double x = lo + m * delta;
+        return in_r ? x : std::pow(10.0, (x - kAbsM) / 5.0 + 1.0) / 1000.0;
+    };
     for (long m = 0; m < Rp * row; m++) {
-        // r(g) = 10^((g - 4.2) / 5 + 1) / 1000; padding entries carry no weight.
-        out->rq[2 * m] = m < M ? std::pow(10.0, (lo + m * delta - kAbsM) / 5.0 + 1.0) / 1000.0 : r_last;
+        // padding entries repeat the last node and carry no weight
+        out->rq[2 * m] = node_r(m < M ? m : M - 1);
         out->rq[2 * m + 1] = m < M ? K[m] : 0.0;
     }
     out->rstep.assign(Rp, RStep{});

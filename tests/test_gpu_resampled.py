@@ -1,7 +1,7 @@
 """The resampled-quadrature mode (MW_PREC_RESAMPLED) on the GPU.
 
-It is mixed precision on a quadrature whose weights are folded onto a
-magnitude grid.  Bars, as for the mixed path: every star's |delta ln P| <= 1e-7
+It is mixed precision on a quadrature whose weights are folded onto a grid
+in distance or magnitude.  Bars, as for the mixed path: every star's |delta ln P| <= 1e-7
 against the FP64 oracle, integrals within 2e-8 relative (the FP32
 transcendentals dominate; the quadrature change itself is ~1e-10).  Cases
 include the narrowest streams the parameter ranges allow (sigma = 1 kpc),
@@ -65,6 +65,19 @@ def test_narrowest_streams_far_out():
     e, grew = errors_vs_oracle(p, stars)
     assert grew
     assert e["lnP"] <= LNP_TOL and e["I_st"] <= INT_REL_TOL and e["I_bg"] <= INT_REL_TOL
+
+
+def test_sight_through_the_galactic_centre():
+    """Stripe 52 near mu = 246 deg passes ~0.16 kpc from the Galactic centre:
+    the background's sharpest peak, where the magnitude grid is the one
+    that fits."""
+    p, stars = synth.workload(98, LONG_R, 2)
+    p.wedge = 52
+    for reg in p.regions:
+        reg.mu_min, reg.mu_max = 236.0, 256.0
+    e, grew = errors_vs_oracle(p, stars)
+    assert grew
+    assert e["lnP"] <= LNP_TOL and e["I_bg"] <= INT_REL_TOL and e["I_st"] <= INT_REL_TOL
 
 
 def test_aux_profile_fixture():

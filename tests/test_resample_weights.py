@@ -40,6 +40,32 @@ def test_polynomials_are_exact(points):
             assert abs(got - want) <= 1e-11 * np.sum(np.abs(w * (g - c) ** j)), (delta, j)
 
 
+def test_distance_grid_polynomials_are_exact(points):
+    rp, _, w = points
+    lo, K = sep.resample_weights(rp, w, 0.25)
+    rm = lo + 0.25 * np.arange(K.size)
+    c = 50.0
+    for j in range(16):
+        want = np.sum(w * (rp - c) ** j)
+        got = np.sum(K * (rm - c) ** j)
+        assert abs(got - want) <= 1e-11 * np.sum(np.abs(w * (rp - c) ** j)), j
+
+
+def test_distance_grid_narrow_stream(points):
+    """A grid uniform in distance at 0.2 sigma: the stream Gaussian's width in
+    distance is at least sigma at any distance."""
+    rp, _, w = points
+    lo, K = sep.resample_weights(rp, w, 0.2)
+    rm = lo + 0.2 * np.arange(K.size)
+    rng = np.random.default_rng(4)
+    k2 = np.log2(np.e) / 2.0           # sigma = 1 kpc
+    for _ in range(300):
+        rc, pu, e = rng.uniform(2, 100), rng.uniform(0.3, 1.0), rng.uniform(0, 3)
+        f = lambda r: np.exp2(-k2 * ((pu * (r - rc)) ** 2 + e))
+        exact = np.sum(w * f(rp))
+        assert abs(np.sum(K * f(rm)) - exact) <= 1e-11 * exact
+
+
 def test_grid_covers_points_with_margin(points):
     _, g, w = points
     lo, K = sep.resample_weights(g, w, 0.01)
