@@ -19,11 +19,12 @@
 //
 // Precision modes: MIXED == false is FP64 throughout (the Gaussian as a
 // polynomial 2^x, 1/sqrt and 1/x from hardware estimates refined to full FP64
-// accuracy).  MIXED == true
-// keeps FP64 for the positions (lbr -> xyz), the squared stream-axis
-// distance, the background's squared radius, every accumulation and the
-// per-star log, and evaluates sqrt, the reciprocal and exp in FP32 on the
-// transcendental unit (v_rsq_f32, v_rcp_f32, v_exp_f32).
+// accuracy).  MIXED == true evaluates the streams' exp in FP32 on the
+// transcendental unit (v_exp_f32) and keeps FP64 for the positions
+// (lbr -> xyz), the squared stream-axis distance, the background (its 1/sqrt
+// and 1/x from the FP64 estimates with one Newton step: as cheap as FP32
+// estimates with their conversions, and exact to FP64), every accumulation
+// and the per-star log.
 #include <hip/hip_runtime.h>
 
 #include "mw_internal.h"
@@ -230,15 +231,7 @@ __device__ __forceinline__ void point_terms(double r, double qw, const Sight<S>&
 {
     if constexpr (BG != kBgZeroQ) {
         const double rg2 = quad_at(v.bg, r);
-        if constexpr (MIXED) {
-            // FP32 estimates of 1/sqrt and 1/x, one FP64 Newton step each
-            double rinv = (double)__builtin_amdgcn_rsqf((float)rg2);
-            rinv = fma(0.5 * rinv, fma(-(rg2 * rinv), rinv, 1.0), rinv);
-            const double rs = fma(rg2, rinv, K.r0);
-            double irs = (double)__builtin_amdgcn_rcpf((float)rs);
-            irs = fma(irs, fma(-rs, irs, 1.0), irs);
-            bg = fma(qw * rinv, irs * irs * irs, bg);
-        } else if constexpr (BG == kBgGeneral) {
+        if constexpr (BG == kBgGeneral) {
             const double rg = sqrt(rg2);
             bg += qw / (pow(rg, K.alpha) * pow(rg + K.r0, K.alpha_delta3));
         } else {
