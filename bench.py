@@ -145,31 +145,49 @@ def full_size_fp64(steps: int, warmup: int, precision=sep.PREC_FP64):
     }
 
 
-def serving_run(n_requests: int, rate: float, variant: int, bpc: int):
+def serving_run(n_requests: int, rates, variant: int, bpc: int):
+    """Poisson arrival streams at each rate over 8 resident wedges; requests
+    are resampled mixed precision with a 10% FP64 share.  The plain
+    one-at-a-time FP64 server is run at the lowest rate only (it saturates
+    at about 10 requests/s)."""
     wedges = serving.make_wedges(8, seed=4000)
     holders = [(ParamsHolder(p), StarsHolder(s)) for p, s, _ in wedges]
-    reqs = serving.make_requests(wedges, n_requests, seed=5000, fast_precision=sep.PREC_RESAMPLED)
-    arr = serving.poisson_arrivals(len(reqs), rate, seed=6000)
-    out = {}
+    out = {"wedges": "8 resident wedges from the batch shapes, 1-4 streams, 10% FP64 requests",
+           "sweep": []}
     with sep.Server(max_inflight=3, max_batch=32, variant=variant, blocks_per_cu=bpc) as srv:
         ids = [srv.add_wedge(ph, sh) for ph, sh in holders]
-        served = serving.run(srv, ids, reqs, arr)
-        st = srv.stats()
+        for k, rate in enumerate(rates):
+            n = max(n_requests, min(400, int(2 * rate)))
+            reqs = serving.make_requests(wedges, n, seed=5000 + k, fast_precision=sep.PREC_RESAMPLED)
+            arr = serving.poisson_arrivals(len(reqs), rate, seed=6000 + k)
+            b0 = srv.stats().batches
+            v = serving.run(srv, ids, reqs, arr)
+            st = srv.stats()
+            row = {"poisson_rate_per_s": rate, "requests": n,
+                   "throughput_req_per_s": round(v["throughput"], 2),
+                   "p50_ms": round(v["p50_ms"], 2), "p99_ms": round(v["p99_ms"], 2),
+                   "batches": int(st.batches - b0)}
+            for prec, name in ((sep.PREC_RESAMPLED, "resampled"), (sep.PREC_FP64, "fp64")):
+                if f"p50_ms_prec{prec}" in v:
+                    row[f"{name}_p50_ms"] = round(v[f"p50_ms_prec{prec}"], 2)
+                    row[f"{name}_p99_ms"] = round(v[f"p99_ms_prec{prec}"], 2)
+                    row[f"{name}_requests"] = v[f"n_prec{prec}"]
+                    row[f"{name}_mean_queue_ms"] = round(v[f"queue_ms_prec{prec}"], 2)
+                    row[f"{name}_mean_service_ms"] = round(v[f"service_ms_prec{prec}"], 2)
+            log(f"  rate {rate}/s: {row}")
+            out["sweep"].append(row)
+        out["largest_batch"] = int(srv.stats().largest_batch)
+    rate = rates[0]
+    reqs = serving.make_requests(wedges, n_requests, seed=5000, fast_precision=sep.PREC_RESAMPLED)
+    arr = serving.poisson_arrivals(len(reqs), rate, seed=6000)
     with sep.Server(max_inflight=1, max_batch=1) as srv:
         ids = [srv.add_wedge(ph, sh) for ph, sh in holders]
         plain = serving.run(srv, ids, reqs, arr, force_precision=sep.PREC_FP64)
-    for k, v in (("continuous_batching_resampled_mixed", served), ("one_at_a_time_fp64", plain)):
-        out[k] = {"throughput_req_per_s": round(v["throughput"], 3),
-                  "p50_ms": round(v["p50_ms"], 2), "p99_ms": round(v["p99_ms"], 2)}
-    out["requests"] = n_requests
-    out["poisson_rate_per_s"] = rate
-    out["batches"] = int(st.batches)
-    out["largest_batch"] = int(st.largest_batch)
-    out["wedges"] = "8 resident wedges from the batch shapes, 1-4 streams, 10% FP64 requests"
+    out["one_at_a_time_fp64"] = {"poisson_rate_per_s": rate, "requests": n_requests,
+                                 "throughput_req_per_s": round(plain["throughput"], 2),
+                                 "p50_ms": round(plain["p50_ms"], 2), "p99_ms": round(plain["p99_ms"], 2)}
     return out
 
-
-# --------------------------------------------------------------------------
 
 def timed_batch(pairs, vecs, precision, choice, steps, warmup):
     """Eager run (kernel events around the persistent launch), then `steps`
@@ -334,7 +352,7 @@ def bench_batch(args):
         out["full_size_fp64"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in full.items()}
     if not args.no_serving:
         log("serving run")
-        out["serving"] = serving_run(args.serving_requests, args.serving_rate, choice["variant"],
+        out["serving"] = serving_run(args.serving_requests, args.serving_rates, choice["variant"],
                                      choice["blocks_per_cu"])
     if not args.no_cpu:
         log("CPU baseline (oracle, 1 thread, bounded sample)")
@@ -408,7 +426,8 @@ def main():
     ap.add_argument("--no-secondary", action="store_true",
                     help="skip the second batch precision (for profiling one path)")
     ap.add_argument("--serving-requests", type=int, default=40)
-    ap.add_argument("--serving-rate", type=float, default=10.0)
+    ap.add_argument("--serving-rates", type=float, nargs="+", default=[10.0, 50.0, 200.0, 800.0],
+                    help="Poisson arrival rates (requests/s) of the serving sweep")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))

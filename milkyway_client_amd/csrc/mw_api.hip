@@ -70,7 +70,6 @@ int build_context(mw_ctx* c, const std::vector<RegionGeom>& geoms, int n_conv, i
         return fail(MW_E_INVALID, "convolve must be between 1 and 1024");
     if (geoms.empty()) return fail(MW_E_INVALID, "no integration regions");
     MW_HIP_CHECK(hipGetDevice(&c->device));
-    MW_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto& e : c->ev) MW_HIP_CHECK(hipEventCreate(&e));
     c->have_events = true;
     c->n_conv = n_conv;
@@ -140,10 +139,11 @@ int run_plan(mw_ctx* c, Plan& P)
     const int NI = integral_comps(), NS = star_comps();
     const int n_slots = (int)P.K.size();
     const int n_pr = (int)P.regions.size();
-    hipStream_t st = c->stream;
     const bool timed = (c->flags & MW_CTX_TIMING) != 0;
     int rc;
     MW_HIP_CHECK(hipSetDevice(c->device));
+    if ((rc = c->ensure_stream())) return rc;
+    hipStream_t st = c->stream;
     if ((rc = c->consts.reserve(n_slots)) || (rc = c->h_consts.reserve(n_slots))) return rc;
     std::memcpy(c->h_consts.p, P.K.data(), sizeof(EvalConst) * n_slots);
 

@@ -88,6 +88,7 @@ private:
     DevBuf<unsigned> counters_;
     HostBuf<EvalConst> h_consts_;
     HostBuf<double> h_irows_, h_srows_;
+    HostBuf<unsigned char> h_plan_;   // staging of the planned work items
 };
 
 }  // namespace mw

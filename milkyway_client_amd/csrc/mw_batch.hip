@@ -63,10 +63,8 @@ size_t BatchRunner::scratch_bytes_for(const std::vector<Slot>& slots)
 
 size_t BatchRunner::scratch_bytes() const
 {
-    return consts_.cap * sizeof(EvalConst) + ritems_.cap * sizeof(RegionLaunch) +
-           sitems_.cap * sizeof(StarLaunch) + (rred_.cap + sred_.cap) * sizeof(ReduceItem) +
-           (ipart_.cap + spart_.cap + irows_.cap + srows_.cap) * sizeof(double) +
-           (first_.cap + count_.cap) * sizeof(int);
+    return consts_.held + ritems_.held + sitems_.held + rred_.held + sred_.held + ipart_.held +
+           spart_.held + irows_.held + srows_.held + first_.held + count_.held + counters_.held;
 }
 
 int BatchRunner::plan(const std::vector<Slot>& slots, int precision)
@@ -202,12 +200,32 @@ int BatchRunner::plan(const std::vector<Slot>& slots, int precision)
         fr[s] = slot_first_row_[s];
         cnt[s] = slot_n_regions_[s];
     }
-    MW_HIP_CHECK(hipMemcpy(ritems_.p, hl.data(), sizeof(RegionLaunch) * n_ritems_, hipMemcpyHostToDevice));
-    MW_HIP_CHECK(hipMemcpy(rred_.p, hr.data(), sizeof(ReduceItem) * n_ritems_, hipMemcpyHostToDevice));
-    MW_HIP_CHECK(hipMemcpy(sitems_.p, hs.data(), sizeof(StarLaunch) * n, hipMemcpyHostToDevice));
-    MW_HIP_CHECK(hipMemcpy(sred_.p, hsr.data(), sizeof(ReduceItem) * n, hipMemcpyHostToDevice));
-    MW_HIP_CHECK(hipMemcpy(first_.p, fr.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-    MW_HIP_CHECK(hipMemcpy(count_.p, cnt.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    // Work items go up through one pinned staging area, asynchronously on the
+    // runner's stream, ahead of the launches that read them: a plan never
+    // waits for work that other runners have in flight.  The staging area is
+    // reused only by the next plan, after this runner's batch has finished.
+    struct Part {
+        void* dst;
+        const void* src;
+        size_t bytes;
+    };
+    const Part parts[] = {
+        { ritems_.p, hl.data(), sizeof(RegionLaunch) * n_ritems_ },
+        { rred_.p, hr.data(), sizeof(ReduceItem) * n_ritems_ },
+        { sitems_.p, hs.data(), sizeof(StarLaunch) * n },
+        { sred_.p, hsr.data(), sizeof(ReduceItem) * n },
+        { first_.p, fr.data(), sizeof(int) * n },
+        { count_.p, cnt.data(), sizeof(int) * n },
+    };
+    size_t total = 0;
+    for (const Part& q : parts) total += (q.bytes + 255) / 256 * 256;
+    if ((rc = h_plan_.reserve(total))) return rc;
+    size_t off = 0;
+    for (const Part& q : parts) {
+        std::memcpy(h_plan_.p + off, q.src, q.bytes);
+        MW_HIP_CHECK(hipMemcpyAsync(q.dst, h_plan_.p + off, q.bytes, hipMemcpyHostToDevice, stream_));
+        off += (q.bytes + 255) / 256 * 256;
+    }
     set_constants(slots);
     return MW_OK;
 }

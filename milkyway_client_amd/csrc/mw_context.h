@@ -59,55 +59,57 @@ int dropin_precision_value();
             return fail(MW_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));       \
     } while (0)
 
-// Device buffer that grows on demand.
-template <typename T>
-struct DevBuf {
+// Buffers that grow on demand.  Freeing device or pinned memory waits for
+// the whole device, including other streams' kernels, so growth never frees:
+// the outgrown block is kept until the buffer itself is released, and a
+// buffer that grows again takes half as much again, which bounds the kept
+// blocks at twice the final size and settles after a few batches.  The first
+// allocation is exact (resident tables are allocated once).
+template <typename T, bool PINNED>
+struct GrowBuf {
     T* p = nullptr;
     size_t cap = 0;
-    ~DevBuf() { release(); }
+    size_t held = 0;   // bytes allocated, outgrown blocks included
+    std::vector<T*> outgrown;
+    GrowBuf() = default;
+    GrowBuf(const GrowBuf&) = delete;
+    GrowBuf& operator=(const GrowBuf&) = delete;
+    ~GrowBuf() { release(); }
+    static void free_block(T* q)
+    {
+        if (!q) return;
+        if (PINNED) (void)hipHostFree(q);
+        else (void)hipFree(q);
+    }
     void release()
     {
-        if (p) (void)hipFree(p);
+        free_block(p);
+        for (T* q : outgrown) free_block(q);
+        outgrown.clear();
         p = nullptr;
         cap = 0;
+        held = 0;
     }
     int reserve(size_t n)
     {
         if (n <= cap) return MW_OK;
-        release();
-        if (hipMalloc(&p, sizeof(T) * std::max<size_t>(n, 1)) != hipSuccess) {
-            p = nullptr;
-            return fail(MW_E_NOMEM, "hipMalloc failed");
-        }
-        cap = n;
+        const size_t want = cap ? std::max(n, cap + cap / 2) : n;
+        const size_t bytes = sizeof(T) * std::max<size_t>(want, 1);
+        T* q = nullptr;
+        const hipError_t e = PINNED ? hipHostMalloc((void**)&q, bytes, hipHostMallocDefault)
+                                    : hipMalloc((void**)&q, bytes);
+        if (e != hipSuccess)
+            return fail(MW_E_NOMEM, PINNED ? "hipHostMalloc failed" : "hipMalloc failed");
+        if (p) outgrown.push_back(p);
+        p = q;
+        cap = want;
+        held += bytes;
         return MW_OK;
     }
 };
 
-// Pinned host buffer that grows on demand.
-template <typename T>
-struct HostBuf {
-    T* p = nullptr;
-    size_t cap = 0;
-    ~HostBuf() { release(); }
-    void release()
-    {
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-    int reserve(size_t n)
-    {
-        if (n <= cap) return MW_OK;
-        release();
-        if (hipHostMalloc((void**)&p, sizeof(T) * std::max<size_t>(n, 1), hipHostMallocDefault) != hipSuccess) {
-            p = nullptr;
-            return fail(MW_E_NOMEM, "hipHostMalloc failed");
-        }
-        cap = n;
-        return MW_OK;
-    }
-};
+template <typename T> using DevBuf = GrowBuf<T, false>;    // device memory
+template <typename T> using HostBuf = GrowBuf<T, true>;    // pinned host memory
 
 // Geometry of one region, as INTEGRAL_AREA holds it.
 struct RegionGeom {
@@ -332,6 +334,19 @@ struct mw_ctx {
     const STAR_POINTS* key_sp = nullptr;
     int key_stars = 0;
     uint64_t key_hash = 0;
+
+    // The context's own stream is created on its first evaluation: a context
+    // that only serves batches (a server's wedge) holds no stream, and with
+    // it no hardware queue that a batch runner's stream might have to share.
+    int ensure_stream()
+    {
+        if (stream) return MW_OK;
+        if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) {
+            stream = nullptr;
+            return mw::fail(MW_E_HIP, "hipStreamCreateWithFlags failed");
+        }
+        return MW_OK;
+    }
 
     ~mw_ctx()
     {

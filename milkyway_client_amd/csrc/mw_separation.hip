@@ -108,6 +108,7 @@ int mw_ctx_separation(mw_ctx* c, const double* params, int precision, const doub
         it.ratio = c->ratio.p;
         it.kw = c->kw.p;
         const bool mixed = precision == MW_PREC_MIXED;
+        if ((rc = c->ensure_stream())) return rc;
         MW_HIP_CHECK(launch_separation(S, K.bg_mode, mixed, it, K, C, c->sep_assign.p, c->sep_xyz.p,
                                        c->sep_prob.p, c->stream));
         int* a_out = assignment;

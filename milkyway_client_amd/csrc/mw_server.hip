@@ -2,13 +2,17 @@
 //
 // Wedges (geometry tables and star lists) are made resident once.  Requests
 // (a wedge, a parameter vector, a precision) are queued as they arrive; a
-// dispatcher thread repeatedly forms a batch from the pending requests,
-// admitted in arrival order while the in-flight scratch fits the HBM budget
-// and a runner (a stream with its own buffers) is free, and launches it.  Up
-// to max_inflight batches run at once on separate streams, so requests that
-// arrive while others are on the GPU are admitted as runners and memory free
-// up and their work overlaps on the device.  Each request's result is
-// bitwise the same alone and in any mix (per-request fixed-order reductions).
+// dispatcher thread repeatedly forms a batch of one precision class from the
+// pending requests (the oldest request's class, oldest first within it),
+// admitted while the in-flight scratch fits the HBM budget and a runner (a
+// stream with its own buffers) is free, and launches it.  Up to max_inflight
+// batches run at once on separate streams, so requests that arrive while
+// others are on the GPU are admitted as runners and memory free up, and
+// their work overlaps on the device.  FP64 batches, far slower per
+// evaluation, hold at most half the runners and run their persistent
+// grid at no more than 2 workgroups per CU, so faster batches find both a
+// runner and free wave slots.  Each request's result is bitwise the same
+// alone and in any mix (per-request fixed-order reductions).
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -81,25 +85,48 @@ struct mw_server {
 
 bool mw_server::launch_locked(InFlight& f, std::unique_lock<std::mutex>& lk)
 {
-    // Admission: requests in arrival order, one precision per batch, up to
-    // max_batch vectors, while the batch's scratch fits the budget.
+    // Admission.  The batch takes one precision class: that of the oldest
+    // pending request, except that FP64 batches (tens to hundreds of times
+    // slower per evaluation) may hold at most half the runners (at least
+    // one), so that faster requests always find one.  Within the class, requests go
+    // in arrival order, up to max_batch vectors, while the batch's scratch
+    // fits the budget; requests of other classes keep their places.
+    int fp64_busy = 0;
+    for (const InFlight& g : slots)
+        if (g.busy && !g.reqs.empty() && g.reqs.front().precision == MW_PREC_FP64) fp64_busy++;
+    const bool allow_fp64 = cfg.max_inflight == 1 || fp64_busy < std::max(1, cfg.max_inflight / 2);
+    int cls = -1;
+    for (const Request& r : pending) {
+        if (r.precision == MW_PREC_FP64 && !allow_fp64) continue;
+        cls = r.precision;
+        break;
+    }
+    if (cls < 0) return false;
     std::vector<Request> take;
     std::vector<Slot> slots_v;
+    std::deque<Request> rest;
     size_t scratch = 0;
     const size_t budget = cfg.hbm_budget_bytes;
-    while (!pending.empty() && (int)take.size() < cfg.max_batch_vectors) {
-        const Request& r = pending.front();
-        if (!take.empty() && r.precision != take.front().precision) break;
+    bool full = false;
+    for (const Request& r : pending) {
+        if (full || r.precision != cls || (int)take.size() >= cfg.max_batch_vectors) {
+            rest.push_back(r);
+            continue;
+        }
         std::vector<Slot> trial = slots_v;
         trial.push_back(Slot{ wedges[r.wedge].get(), r.k });
         const size_t need = BatchRunner::scratch_bytes_for(trial);
-        if (budget && resident + inflight_scratch + need > budget) break;
+        if (budget && resident + inflight_scratch + need > budget) {
+            full = true;   // later requests of the class wait behind this one
+            rest.push_back(r);
+            continue;
+        }
         slots_v.swap(trial);
         scratch = need;
         take.push_back(r);
-        pending.pop_front();
     }
     if (take.empty()) return false;
+    pending.swap(rest);
     f.reqs = std::move(take);
     f.scratch = scratch;
     f.busy = true;
@@ -109,6 +136,10 @@ bool mw_server::launch_locked(InFlight& f, std::unique_lock<std::mutex>& lk)
     max_batch_seen = std::max<long>(max_batch_seen, (long)f.reqs.size());
     const int precision = f.reqs.front().precision;
     lk.unlock();
+    // An FP64 batch's persistent grid takes at most 2 workgroups per CU,
+    // leaving wave slots on every CU to a concurrent faster batch.
+    const int bpc = cfg.blocks_per_cu > 0 ? cfg.blocks_per_cu : 2;
+    f.runner->set_variant(cfg.variant, precision == MW_PREC_FP64 ? std::min(bpc, 2) : bpc);
     int rc = hipSetDevice(device) == hipSuccess ? MW_OK : MW_E_HIP;
     if (!rc) rc = f.runner->plan(slots_v, precision);
     if (!rc) rc = f.runner->enqueue();

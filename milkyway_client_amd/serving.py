@@ -70,15 +70,19 @@ def run(server: sep.Server, wedge_ids: Sequence[int], requests: Sequence[Request
             time.sleep(delay)
         prec = req.precision if force_precision is None else force_precision
         ids.append(server.submit(wedge_ids[req.wedge], req.vector, prec))
-    lat, first, last, results = [], None, None, []
-    for rid in ids:
+    lat, first, last, results, precs, queued = [], None, None, [], [], []
+    for req, rid in zip(requests, ids):
         res, (ts, td, tdone) = server.wait(rid)
         lat.append(tdone - ts)
+        queued.append(td - ts)
         first = ts if first is None else min(first, ts)
         last = tdone if last is None else max(last, tdone)
         results.append(res)
+        precs.append(req.precision if force_precision is None else force_precision)
     lat = np.asarray(lat)
-    return {
+    queued = np.asarray(queued)
+    precs = np.asarray(precs)
+    out = {
         "n": len(ids),
         "throughput": len(ids) / (last - first) if last > first else float("nan"),
         "p50_ms": 1e3 * float(np.percentile(lat, 50)),
@@ -86,3 +90,13 @@ def run(server: sep.Server, wedge_ids: Sequence[int], requests: Sequence[Request
         "mean_ms": 1e3 * float(lat.mean()),
         "results": results,
     }
+    # the same percentiles per precision class
+    for prec in sorted(set(precs.tolist())):
+        sel = lat[precs == prec]
+        out[f"p50_ms_prec{prec}"] = 1e3 * float(np.percentile(sel, 50))
+        out[f"p99_ms_prec{prec}"] = 1e3 * float(np.percentile(sel, 99))
+        out[f"n_prec{prec}"] = int(sel.size)
+        # split: waiting for admission vs dispatched until collected
+        out[f"queue_ms_prec{prec}"] = 1e3 * float(queued[precs == prec].mean())
+        out[f"service_ms_prec{prec}"] = 1e3 * float((sel - queued[precs == prec]).mean())
+    return out
