@@ -20,7 +20,9 @@
 // Precision modes: MIXED == false is FP64 throughout (the Gaussian as a
 // polynomial 2^x, 1/sqrt and 1/x from hardware estimates refined to full FP64
 // accuracy).  MIXED == true evaluates the streams' exp in FP32 on the
-// transcendental unit (v_exp_f32) and keeps FP64 for the positions
+// transcendental unit (v_exp_f32; in the integral pass of 2^-t^2 with the
+// sight's minimum exponent applied per column in FP64, in the star pass of
+// the fractional part of the exponent) and keeps FP64 for the positions
 // (lbr -> xyz), the squared stream-axis distance, the background (its 1/sqrt
 // and 1/x from the FP64 estimates with one Newton step: as cheap as FP32
 // estimates with their conversions, and exact to FP64), every accumulation
@@ -225,7 +227,14 @@ __device__ __forceinline__ double quad_at(const Quad& q, double r)
 // qw = W_i r^3 N(dx_i)), accumulated into bg and st[]
 // (evaluation_optimized.c:222-296 per point).
 // ---------------------------------------------------------------------------
-template <int S, int BG, bool MIXED>
+//
+// SHIFTED (mixed precision, integral pass): each stream's exponent is taken
+// without the sight's minimum e, a = t^2, and the caller multiplies the
+// column's sums by 2^-e in FP64 once.  The FP32 exponent then rounds t^2
+// rather than t^2 + e (relative error of the term |t^2| 2^-24 ln 2, unbiased
+// over the points), and terms below 2^-126 of the sight's own peak flush to
+// zero while the range of 2^-e stays FP64's.
+template <int S, int BG, bool MIXED, bool SHIFTED = false>
 __device__ __forceinline__ void point_terms(double r, double qw, const Sight<S>& v,
                                             const EvalConst& K, double& bg, double (&st)[kMaxStreams])
 {
@@ -242,6 +251,12 @@ __device__ __forceinline__ void point_terms(double r, double qw, const Sight<S>&
     }
 #pragma unroll
     for (int s = 0; s < S; s++) {
+        if constexpr (MIXED && SHIFTED) {
+            const double t = fma(v.st[s].p, r, -v.st[s].m);
+            const float e = __builtin_amdgcn_exp2f(-(float)(t * t));
+            st[s] = fma(qw, (double)e, st[s]);
+            continue;
+        }
         const double a = quad_at(v.st[s], r);
         if constexpr (MIXED) {
             // 2^t with t = -a split in FP64 into n + f, f in [0, 1): only 2^f
@@ -309,7 +324,7 @@ __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
 #pragma unroll UNROLL
                 for (int i = 0; i < n; i++) {
                     const double2 p = row[i];
-                    point_terms<S, BG, MIXED>(p.x, p.y, sight, K, bg, st);
+                    point_terms<S, BG, MIXED, MIXED>(p.x, p.y, sight, K, bg, st);
                 }
                 const RStep& rs = it.rstep[k];
                 double f = rs.reff_xr_rp3 * rs.irv;
@@ -331,7 +346,11 @@ __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
         if (valid) {
             out[0] = DD{ (acc[0] + cmp[0]) * wnu, 0.0 };
 #pragma unroll
-            for (int s = 0; s < S; s++) out[1 + s] = DD{ (acc[1 + s] + cmp[1 + s]) * wnu * K.on[s], 0.0 };
+            for (int s = 0; s < S; s++) {
+                double v = (acc[1 + s] + cmp[1 + s]) * wnu * K.on[s];
+                if constexpr (MIXED) v *= exp2_neg(sight.st[s].e);   // the minimum taken out per point
+                out[1 + s] = DD{ v, 0.0 };
+            }
         }
     }
     block_reduce_dd<kIntComps>(out, red);
