@@ -97,6 +97,29 @@ def cpu_seconds_per_eval(p, stars, rows_per_region, n_star_sample):
 # Workloads
 # --------------------------------------------------------------------------
 
+def mixed_roofline(points: float, rec: float, n_streams: int, ms: float) -> dict:
+    """Roofline of a mixed-precision integral launch, each evaluation priced
+    by the method that ran it: the frozen per-point model (FP32 exp, against
+    the FP32 vector peak) for exponentials per point, the same weights over
+    the recurrence's operations (FP64, against the FP64 peak) on distance
+    grids.  frac is the time the launch would take at those peaks over its
+    measured time."""
+    w_exp = (points - rec) * workmodel.flops_per_point(n_streams)
+    w_rec = rec * workmodel.flops_per_point_recurrence(n_streams)
+    t_peak = (w_exp / FP32_PEAK_TFLOPS + w_rec / FP64_PEAK_TFLOPS) / 1e12
+    work = w_exp + w_rec
+    return {"achieved": round(work / (ms * 1e-3) / 1e12, 3),
+            "peak": round(work / t_peak / 1e12, 2) if t_peak > 0 else FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(t_peak / (ms * 1e-3), 4),
+            "integrand_evaluations_per_launch": int(points),
+            "of_which_by_recurrence": int(rec),
+            "algorithmic_flop_eq_per_launch": work,
+            "pricing": (f"{workmodel.flops_per_point(n_streams)} flop-eq per evaluation with an exp "
+                        f"per stream at {FP32_PEAK_TFLOPS} TFLOP/s (FP32), "
+                        f"{workmodel.flops_per_point_recurrence(n_streams)} by recurrence at "
+                        f"{FP64_PEAK_TFLOPS} (FP64); peak is their work-weighted mix")}
+
+
 def build_batch():
     groups, vecs, plist = [], [], []
     for k, shape in enumerate(synth.BATCH_SHAPES):
@@ -118,7 +141,7 @@ def full_size_fp64(steps: int, warmup: int, precision=sep.PREC_FP64):
     vecs = synth.parameter_vectors(1, p, 4)
     for i in range(warmup):
         ctx.eval(vecs[i % 4], precision=precision)
-    lat, kern, star, points = [], [], [], []
+    lat, kern, star, points, rec = [], [], [], [], []
     t0 = time.perf_counter()
     for i in range(steps):
         s0 = time.perf_counter()
@@ -128,6 +151,7 @@ def full_size_fp64(steps: int, warmup: int, precision=sep.PREC_FP64):
         kern.append(tm.integral_ms)
         star.append(tm.star_ms)
         points.append(tm.integral_points)
+        rec.append(tm.integral_points_recurrence)
     el = time.perf_counter() - t0
     ctx.close()
     # Integrand evaluations of the integral pass: cells x n for the exact
@@ -140,6 +164,7 @@ def full_size_fp64(steps: int, warmup: int, precision=sep.PREC_FP64):
         "achieved_tflops": round(flops / (k_ms * 1e-3) / 1e12, 3),
         "frac_fp64_peak": round(flops / (k_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
         "integrand_evaluations": int(np.mean(points)), "flop_eq": flops,
+        "integrand_evaluations_by_recurrence": int(np.mean(rec)),
         "resident_setup_s": round(setup, 3),
         "workload": "700x1600x640 + 700x400x160 cut (r x mu x nu), n=120, 3 streams, 200k stars",
     }
@@ -254,7 +279,7 @@ def bench_batch(args):
     log(f"batched mixed precision, headline ({args.batch_precision})")
     value, lat, elapsed, tm = timed_batch(pairs, vecs, head_prec, choice, args.steps, args.warmup)
     int_ms, points = tm.integral_ms, tm.integral_points
-    flops = float(points) * workmodel.flops_per_point(BATCH_STREAMS)
+    roof = mixed_roofline(float(points), float(tm.integral_points_recurrence), BATCH_STREAMS, int_ms)
     log(f"  eager: persistent integral launch {int_ms:.2f} ms of {tm.total_ms:.2f} ms, "
         f"{points:.3e} integrand evaluations")
 
@@ -282,8 +307,9 @@ def bench_batch(args):
 
     quad = "resampled" if head_prec == sep.PREC_RESAMPLED else "exact"
     quad_desc = ("resampled: each region's r-step x convolution weights folded by 16-node Lagrange "
-                 "interpolation onto a magnitude grid sized to the model's narrowest feature "
-                 "(MW_PREC_RESAMPLED); star pass exact" if quad == "resampled" else
+                 "interpolation onto a grid in distance or magnitude sized to the model's narrowest "
+                 "feature (MW_PREC_RESAMPLED); on distance grids the stream Gaussians by FP64 "
+                 "recurrence; star pass exact" if quad == "resampled" else
                  "exact: the reference's r-step x convolution points (MW_PREC_MIXED)")
     out = {
         "metric": "likelihood evals/s, batched mixed precision vs plain FP64 @ per-star |d lnP| <= 1e-7",
@@ -311,37 +337,27 @@ def bench_batch(args):
         "plain_fp64": {"value": round(plain, 4), "unit": "evals/s",
                        "ratio_batched_mixed_over_plain_fp64": round(value / plain, 3),
                        "what": "the same 64 vectors, one evaluation at a time, FP64, exact quadrature"},
-        "roofline": {
-            "bound": "valu", "kernel": f"integral_persistent (mixed, {quad} quadrature)",
-            "achieved": round(flops / (int_ms * 1e-3) / 1e12, 3), "peak": FP32_PEAK_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": round(flops / (int_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
-            "traffic": measured_traffic(f"integral_persistent_{args.batch_precision}_batch64"),
-            "integrand_evaluations_per_launch": int(points),
-            "algorithmic_flop_eq_per_launch": flops,
-            "avg_launch_ms": round(int_ms, 3),
-        },
+        "roofline": dict(
+            {"bound": "valu", "kernel": f"integral_persistent (mixed, {quad} quadrature)"},
+            **roof,
+            traffic=measured_traffic(f"integral_persistent_{args.batch_precision}_batch64"),
+            avg_launch_ms=round(int_ms, 3)),
         "tuning": {"seconds": round(tune_s, 2), "table": choice["table"]},
     }
     if second:
         v2, lat2, tm2, tune2 = second
         q2 = "exact" if quad == "resampled" else "resampled"
-        pts2 = float(tm2.integral_points)
-        f2 = pts2 * workmodel.flops_per_point(BATCH_STREAMS)
+        roof2 = mixed_roofline(float(tm2.integral_points), float(tm2.integral_points_recurrence),
+                               BATCH_STREAMS, tm2.integral_ms)
         key2 = "mixed" if q2 == "exact" else "resampled"
         out[f"batched_mixed_{q2}_quadrature"] = {
             "value": round(v2, 4), "unit": "evals/s", "ratio_over_plain_fp64": round(v2 / plain, 3),
             "accuracy_vs_fp64": accuracy_vs_fp64(groups, vecs, other_prec, per_wedge=1),
             "p50_ms": round(1e3 * float(np.percentile(lat2, 50)), 2),
             "kernel_variant": tune2["variant"], "blocks_per_cu": tune2["blocks_per_cu"],
-            "roofline": {"bound": "valu", "kernel": f"integral_persistent (mixed, {q2} quadrature)",
-                         "achieved": round(f2 / (tm2.integral_ms * 1e-3) / 1e12, 3),
-                         "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                         "frac": round(f2 / (tm2.integral_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
-                         "traffic": measured_traffic(f"integral_persistent_{key2}_batch64"),
-                         "integrand_evaluations_per_launch": int(pts2),
-                         "algorithmic_flop_eq_per_launch": f2,
-                         "avg_launch_ms": round(tm2.integral_ms, 3)},
+            "roofline": dict({"bound": "valu", "kernel": f"integral_persistent (mixed, {q2} quadrature)"},
+                             **roof2, traffic=measured_traffic(f"integral_persistent_{key2}_batch64"),
+                             avg_launch_ms=round(tm2.integral_ms, 3)),
         }
     for ctx, *_ in groups:
         ctx.close()
@@ -389,15 +405,18 @@ def bench_full(args):
                    "batch": 1, "parallelism": "single GPU"},
         "star_kernel_ms": full["star_kernel_ms"],
         "resident_setup_s": full["resident_setup_s"],
-        "roofline": {"bound": "valu", "kernel": "integral_kernel", "achieved": full["achieved_tflops"],
-                     "peak": FP64_PEAK_TFLOPS if prec == sep.PREC_FP64 else FP32_PEAK_TFLOPS,
-                     "unit": "TFLOP/s",
-                     "frac": round(full["achieved_tflops"] / (FP64_PEAK_TFLOPS if prec == sep.PREC_FP64
-                                                              else FP32_PEAK_TFLOPS), 4),
-                     "traffic": (measured_traffic("integral_kernel_fp64_full")
-                                 if prec == sep.PREC_FP64 else None),
-                     "algorithmic_flop_eq_per_launch": flops,
-                     "avg_launch_ms": full["integral_kernel_ms"]},
+        "roofline": ({"bound": "valu", "kernel": "integral_kernel", "achieved": full["achieved_tflops"],
+                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                      "frac": round(full["achieved_tflops"] / FP64_PEAK_TFLOPS, 4),
+                      "traffic": measured_traffic("integral_kernel_fp64_full"),
+                      "algorithmic_flop_eq_per_launch": flops,
+                      "avg_launch_ms": full["integral_kernel_ms"]}
+                     if prec == sep.PREC_FP64 else
+                     dict({"bound": "valu", "kernel": "integral_kernel"},
+                          **mixed_roofline(float(full["integrand_evaluations"]),
+                                           float(full["integrand_evaluations_by_recurrence"]),
+                                           synth.FULL_SIZE_STREAMS, full["integral_kernel_ms"]),
+                          traffic=None, avg_launch_ms=full["integral_kernel_ms"])),
     }
     if not args.no_cpu:
         s, spent = cpu_seconds_per_eval(p, stars, 4, 4000)

@@ -126,6 +126,9 @@ typedef struct {
     long integral_points;              /* integrand evaluations of the integral pass: cells x
                                           convolve points, or sights x grid nodes where the
                                           quadrature is resampled */
+    long integral_points_recurrence;   /* of those, the ones on grids uniform in distance whose
+                                          stream terms came from the FP64 recurrence instead of
+                                          an exponential per point (resampled only) */
 } mw_timing;
 
 /* Flags for mw_ctx_create */

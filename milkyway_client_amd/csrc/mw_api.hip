@@ -151,7 +151,7 @@ int run_plan(mw_ctx* c, Plan& P)
     struct Group { int bg; int first_item, n_items, first_block, n_blocks; };
     std::vector<Group> igroups;
     int total_iblocks = 0;
-    long points = 0;
+    long points = 0, points_rec = 0;
     if (P.integrals && n_pr > 0) {
         if ((rc = c->ritems.reserve(n_pr)) || (rc = c->h_ritems.reserve(n_pr)) ||
             (rc = c->rred.reserve(n_pr)) || (rc = c->h_rred.reserve(n_pr)) ||
@@ -196,6 +196,8 @@ int run_plan(mw_ctx* c, Plan& P)
             L.block_begin = g.n_blocks;          // block index within the group's launch
             L.block_count = rd.blocks;
             L.n_streams = c->S;
+            L.grid_step = recurrence_step(level, P.K[pr.slot]);
+            if (L.grid_step > 0.0) points_rec += rd.n_cols * tr.nodes;
             ReduceItem& R = c->h_rred.p[item];
             R.block_begin = total_iblocks;       // global partial index
             R.block_count = rd.blocks;
@@ -332,6 +334,7 @@ int run_plan(mw_ctx* c, Plan& P)
     }
     c->timing.integral_blocks = total_iblocks;
     c->timing.integral_points = points;
+    c->timing.integral_points_recurrence = points_rec;
     c->timing.star_blocks = total_sblocks;
     return MW_OK;
 }

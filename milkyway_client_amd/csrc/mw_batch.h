@@ -59,6 +59,7 @@ public:
     hipStream_t stream() const { return stream_; }
     long integral_blocks() const { return total_iblocks_; }
     long integral_points() const { return points_; }
+    long integral_points_recurrence() const { return points_rec_; }
     long star_blocks() const { return total_sblocks_; }
 
     static size_t scratch_bytes_for(const std::vector<Slot>& slots);
@@ -75,6 +76,7 @@ private:
     int variant_ = 0, blocks_per_cu_ = 2;
     std::vector<int> levels_;        // quadrature level per (slot, region), slot-major
     long points_ = 0;                // integrand evaluations of the planned integral pass
+    long points_rec_ = 0;            // of those, on distance grids by recurrence
     std::vector<int> slot_bg_, slot_n_regions_, slot_first_row_, slot_S_;
     std::vector<Group> igroups_, sgroups_;
     int n_ritems_ = 0, total_iblocks_ = 0, total_sblocks_ = 0;

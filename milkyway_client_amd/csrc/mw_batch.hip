@@ -76,6 +76,7 @@ int BatchRunner::plan(const std::vector<Slot>& slots, int precision)
     resample_ = precision == MW_PREC_RESAMPLED;
     levels_.clear();
     points_ = 0;
+    points_rec_ = 0;
     slot_bg_.assign(n, 0);
     slot_n_regions_.assign(n, 0);
     slot_first_row_.assign(n, 0);
@@ -118,6 +119,8 @@ int BatchRunner::plan(const std::vector<Slot>& slots, int precision)
             L.out_index = row++;
             L.block_count = rd->blocks;
             L.n_streams = slots[s].k.n_streams;
+            L.grid_step = recurrence_step(level, slots[s].k);
+            if (L.grid_step > 0.0) points_rec_ += rd->n_cols * tr.nodes;
             items.push_back(It{ L, (double)L.r_steps * L.n_conv * (2.0 + L.n_streams) });
         }
     }
@@ -470,6 +473,7 @@ int mw_batch_run(mw_batch* b, const double* params, mw_result* results)
     b->timing.star_ms = 0;
     b->timing.integral_blocks = b->runner.integral_blocks();
     b->timing.integral_points = b->runner.integral_points();
+    b->timing.integral_points_recurrence = b->runner.integral_points_recurrence();
     b->timing.star_blocks = b->runner.star_blocks();
     return MW_OK;
 }

@@ -289,6 +289,11 @@ struct TableRef {
 // the exact tables (resampling off, a background mode it does not cover, or
 // no gain).  Pure: builds nothing.
 int select_level(const RegionDev& rd, const EvalConst& k, bool resample);
+
+// Node spacing for the kernels' stream recurrence on table `level`: the
+// distance grid's spacing h when every stream's step p h <= k_s h stays
+// within 1/2, else 0 (evaluate each node's exponential).
+double recurrence_step(int level, const EvalConst& k);
 // The tables of that level, built and uploaded on first use.
 int region_tables(RegionDev& rd, int level, TableRef* out);
 

@@ -81,6 +81,8 @@ struct RegionLaunch {
     int out_index;             // index into the per-region result rows
     int block_begin, block_count;
     int n_streams;             // streams of the slot's model (batched kernels)
+    double grid_step;          // > 0: rows of nodes uniform in distance at this spacing
+                               // (mixed kernels take the stream Gaussians by recurrence)
 };
 
 // Stars resident on the device, SoA.

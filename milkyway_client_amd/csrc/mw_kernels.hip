@@ -234,12 +234,11 @@ __device__ __forceinline__ double quad_at(const Quad& q, double r)
 // rather than t^2 + e (relative error of the term |t^2| 2^-24 ln 2, unbiased
 // over the points), and terms below 2^-126 of the sight's own peak flush to
 // zero while the range of 2^-e stays FP64's.
-template <int S, int BG, bool MIXED, bool SHIFTED = false>
-__device__ __forceinline__ void point_terms(double r, double qw, const Sight<S>& v,
-                                            const EvalConst& K, double& bg, double (&st)[kMaxStreams])
+template <int BG>
+__device__ __forceinline__ void bg_point(double r, double qw, const Quad& q, const EvalConst& K, double& bg)
 {
     if constexpr (BG != kBgZeroQ) {
-        const double rg2 = quad_at(v.bg, r);
+        const double rg2 = quad_at(q, r);
         if constexpr (BG == kBgGeneral) {
             const double rg = sqrt(rg2);
             bg += qw / (pow(rg, K.alpha) * pow(rg + K.r0, K.alpha_delta3));
@@ -249,6 +248,13 @@ __device__ __forceinline__ void point_terms(double r, double qw, const Sight<S>&
             bg = fma(qw * rinv, irs * irs * irs, bg);
         }
     }
+}
+
+template <int S, int BG, bool MIXED, bool SHIFTED = false>
+__device__ __forceinline__ void point_terms(double r, double qw, const Sight<S>& v,
+                                            const EvalConst& K, double& bg, double (&st)[kMaxStreams])
+{
+    bg_point<BG>(r, qw, v.bg, K, bg);
 #pragma unroll
     for (int s = 0; s < S; s++) {
         if constexpr (MIXED && SHIFTED) {
@@ -268,6 +274,55 @@ __device__ __forceinline__ void point_terms(double r, double qw, const Sight<S>&
             st[s] = fma(qw, ldexp((double)e, (int)n), st[s]);
         } else {
             st[s] = fma(qw, exp2_neg(a), st[s]);
+        }
+    }
+}
+
+// Rows of a grid uniform in distance (nodes r_j = r_0 + j h along the whole
+// column), mixed precision, shifted stream exponents.  With t_j = t_0 + j d,
+// d = p h, the terms G_j = 2^-(t_j^2) obey
+//     G_{j+1} = G_j R_j,   R_{j+1} = R_j C,
+// R_0 = 2^-(2 t_0 d + d^2) and C = 2^-(2 d^2) (one per sight): two FP64
+// multiplies per stream and node instead of an exponential.  The state runs
+// on from row to row and is restarted from the FP64 2^x
+//   * every kRestartRows rows, which bounds the accumulated error by
+//     (8 x 32)^2 / 2 ulp = 4e-12 relative, and
+//   * at a row that starts before the Gaussian's peak (t_0 < 0) with
+//     G < 2^-1000: such a G may have lost precision to underflow, and the
+//     terms are about to grow.
+// Past the peak the terms only shrink, so underflow there loses nothing.  The
+// caller guarantees d <= 1/2; t_0 is clamped at -40 on restart, so R stays
+// below 2^40, and a clamped start gives G = 2^-1600 = 0, which stays 0 until
+// the next restart (the terms it stands for are below 2^-(40 - 15.5)^2).
+constexpr int kRestartRows = 8;
+
+template <int S, int BG>
+__device__ __forceinline__ void row_by_recurrence(const double2* row, int n, double h, bool restart,
+                                                  const Sight<S>& v, const double (&C)[S > 0 ? S : 1],
+                                                  double (&G)[S > 0 ? S : 1],
+                                                  double (&Rr)[S > 0 ? S : 1], const EvalConst& K,
+                                                  double& bg, double (&st)[kMaxStreams])
+{
+    const double r0 = row[0].x;
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const double t0 = fma(v.st[s].p, r0, -v.st[s].m);
+        if (restart || (t0 < 0.0 && G[s] < 0x1p-1000)) {
+            const double tc = fmax(t0, -40.0);
+            const double d = v.st[s].p * h;
+            G[s] = exp2_neg(tc * tc);
+            Rr[s] = exp2_neg(fma(2.0 * tc, d, d * d));
+        }
+    }
+#pragma unroll 2
+    for (int i = 0; i < n; i++) {
+        const double2 p = row[i];
+        bg_point<BG>(p.x, p.y, v.bg, K, bg);
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            st[s] = fma(p.y, G[s], st[s]);
+            G[s] *= Rr[s];
+            Rr[s] *= C[s];
         }
     }
 }
@@ -309,6 +364,16 @@ __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
 #pragma unroll
         for (int c = 0; c <= S; c++) acc[c] = cmp[c] = 0.0;
 
+        // Uniform distance grid: the streams by recurrence along each row.
+        const double h = MIXED ? it.grid_step : 0.0;
+        double Cs[S > 0 ? S : 1], G[S > 0 ? S : 1], Rr[S > 0 ? S : 1];
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            const double d = sight.st[s].p * h;
+            Cs[s] = exp2_neg(2.0 * d * d);
+            G[s] = Rr[s] = 0.0;
+        }
+
         const int rt = kTileD2 / n;   // r-steps per LDS tile (host guarantees n <= kTileD2)
         const double2* rq = reinterpret_cast<const double2*>(it.rq);
         for (int k0 = 0; k0 < R; k0 += rt) {
@@ -321,10 +386,15 @@ __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
                 double bg = 0.0;
                 double st[kMaxStreams] = { 0.0, 0.0, 0.0, 0.0 };
                 const double2* row = tile + kk * n;
+                if (MIXED && h > 0.0) {
+                    row_by_recurrence<S, BG>(row, n, h, k % kRestartRows == 0, sight, Cs, G, Rr, K, bg,
+                                             st);
+                } else {
 #pragma unroll UNROLL
-                for (int i = 0; i < n; i++) {
-                    const double2 p = row[i];
-                    point_terms<S, BG, MIXED, MIXED>(p.x, p.y, sight, K, bg, st);
+                    for (int i = 0; i < n; i++) {
+                        const double2 p = row[i];
+                        point_terms<S, BG, MIXED, MIXED>(p.x, p.y, sight, K, bg, st);
+                    }
                 }
                 const RStep& rs = it.rstep[k];
                 double f = rs.reff_xr_rp3 * rs.irv;

@@ -88,6 +88,15 @@ int select_level(const RegionDev& rd, const EvalConst& k, bool resample)
     return t;
 }
 
+double recurrence_step(int level, const EvalConst& k)
+{
+    if (level < kQuadLevels || level >= kQuadTables) return 0.0;
+    const double h = table_delta(level);
+    for (int s = 0; s < k.n_streams; s++)
+        if (!(k.ks[s] * h <= 0.5)) return 0.0;
+    return h;
+}
+
 int region_tables(RegionDev& rd, int level, TableRef* out)
 {
     if (level < 0) {

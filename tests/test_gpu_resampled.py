@@ -80,6 +80,30 @@ def test_sight_through_the_galactic_centre():
     assert e["lnP"] <= LNP_TOL and e["I_bg"] <= INT_REL_TOL and e["I_st"] <= INT_REL_TOL
 
 
+def test_distance_grid_recurrence():
+    """On wedges where the grid uniform in distance is chosen, the stream
+    Gaussians come from the FP64 recurrence: the timing counters say so, and
+    the stream integrals then match the oracle to the quadrature's own error
+    (far below what an FP32 exponential per point gives)."""
+    for seed, streams, sigma in ((121, 4, None), (122, 2, 1.0)):
+        p, stars = synth.workload(seed, LONG_R, streams)
+        if sigma:
+            for st in p.streams:
+                st.parameters[4] = sigma
+        ph, sh = ParamsHolder(p), StarsHolder(stars)
+        with sep.Context(ph, sh, timing=True) as ctx:
+            got = ctx.eval(p.parameter_vector(), precision=sep.PREC_RESAMPLED, per_star=True)[0]
+            tm = ctx.timing()
+        ref = O.evaluate_full(ParamsHolder(p), StarsHolder(stars))
+        e_st = rel(got.stream_integrals, ref.stream_integrals)
+        lnp = math.log(10) * np.max(np.abs(got.log10_p - ref.log10_p))
+        print(f"seed {seed}: {tm.integral_points_recurrence} of {tm.integral_points} by recurrence, "
+              f"I_st {e_st:.2e} lnP {lnp:.2e}")
+        assert tm.integral_points_recurrence > 0.5 * tm.integral_points
+        assert e_st <= 1e-11
+        assert lnp <= LNP_TOL
+
+
 def test_aux_profile_fixture():
     p, stars = fixture("86", n_stars=1500, aux=True)
     e, _ = errors_vs_oracle(p, stars)
