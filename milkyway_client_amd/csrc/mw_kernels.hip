@@ -301,7 +301,7 @@ __device__ __forceinline__ void row_by_recurrence(const double2* row, int n, dou
                                                   const Sight<S>& v, const double (&C)[S > 0 ? S : 1],
                                                   double (&G)[S > 0 ? S : 1],
                                                   double (&Rr)[S > 0 ? S : 1], const EvalConst& K,
-                                                  double& bg, double (&st)[kMaxStreams])
+                                                  double (&acc)[S + 1])
 {
     const double r0 = row[0].x;
 #pragma unroll
@@ -317,12 +317,85 @@ __device__ __forceinline__ void row_by_recurrence(const double2* row, int n, dou
 #pragma unroll 2
     for (int i = 0; i < n; i++) {
         const double2 p = row[i];
-        bg_point<BG>(p.x, p.y, v.bg, K, bg);
+        bg_point<BG>(p.x, p.y, v.bg, K, acc[0]);
 #pragma unroll
         for (int s = 0; s < S; s++) {
-            st[s] = fma(p.y, G[s], st[s]);
+            acc[1 + s] = fma(p.y, G[s], acc[1 + s]);
             G[s] *= Rr[s];
             Rr[s] *= C[s];
+        }
+    }
+}
+
+// One logical block of a resampled region on a distance grid (grid_step > 0;
+// the region is evaluated whole): the streams by recurrence.  The resampled
+// tables carry every weight in the node entries (each row's irv and reff
+// are 1), so the column sums take the terms directly, plainly summed (a
+// column has a few hundred nodes, not the exact quadrature's tens of
+// thousands of points); each stream's 2^-e is taken from the column geometry
+// again at the end rather than held through the loop.  Both keep the
+// register count within the occupancy targets.
+template <int S, int BG>
+__device__ __forceinline__ void integral_block_recurrence(const RegionLaunch& it, long lb,
+                                                          const EvalConst* __restrict__ consts,
+                                                          double* __restrict__ prow, double2* tile,
+                                                          DD (*red)[kIntComps])
+{
+    const EvalConst& K = consts[it.slot];
+    const int R = it.r_steps, n = it.n_conv;
+    const long C = it.n_cols;
+    const long col = lb * kBlock + threadIdx.x;
+    const bool valid = col < C;
+    const long cc = valid ? col : C - 1;
+    const double h = it.grid_step;
+
+    double acc[1 + S];
+#pragma unroll
+    for (int c = 0; c <= S; c++) acc[c] = 0.0;
+    {
+        const Sight<S> sight = make_sight<S, BG>(it.col[cc], it.col[C + cc], it.col[2 * C + cc], K);
+        double Cs[S > 0 ? S : 1], G[S > 0 ? S : 1], Rr[S > 0 ? S : 1];
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            const double d = sight.st[s].p * h;
+            Cs[s] = exp2_neg(2.0 * d * d);
+            G[s] = Rr[s] = 0.0;
+        }
+        const int rt = kTileD2 / n;
+        const double2* rq = reinterpret_cast<const double2*>(it.rq);
+        for (int k0 = 0; k0 < R; k0 += rt) {
+            const int kt = (R - k0 < rt) ? R - k0 : rt;
+            __syncthreads();
+            for (int e = threadIdx.x; e < kt * n; e += kBlock) tile[e] = rq[(long)k0 * n + e];
+            __syncthreads();
+            for (int kk = 0; kk < kt; kk++) {
+                const int k = k0 + kk;
+                row_by_recurrence<S, BG>(tile + kk * n, n, h, k % kRestartRows == 0, sight, Cs, G, Rr,
+                                         K, acc);
+                if constexpr (BG == kBgHernquistAux) {
+                    const RStep& rs = it.rstep[k];
+                    acc[0] += K.bg_a * rs.q2 + K.bg_b * rs.q1 + K.bg_c * rs.q0;
+                }
+            }
+        }
+    }
+    DD out[kIntComps];
+#pragma unroll
+    for (int c = 0; c < kIntComps; c++) out[c] = DD{ 0.0, 0.0 };
+    if (valid) {
+        const double wnu = it.col[3 * C + cc];
+        const Sight<S> again = make_sight<S, BG>(it.col[cc], it.col[C + cc], it.col[2 * C + cc], K);
+        out[0] = DD{ acc[0] * wnu, 0.0 };
+#pragma unroll
+        for (int s = 0; s < S; s++)
+            out[1 + s] = DD{ acc[1 + s] * wnu * K.on[s] * exp2_neg(again.st[s].e), 0.0 };
+    }
+    block_reduce_dd<kIntComps>(out, red);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int c = 0; c < kIntComps; c++) {
+            prow[2 * c] = out[c].hi;
+            prow[2 * c + 1] = out[c].lo;
         }
     }
 }
@@ -337,6 +410,12 @@ __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
                                                double* __restrict__ prow, double2* tile,
                                                DD (*red)[kIntComps])
 {
+    if constexpr (MIXED) {
+        if (it.grid_step > 0.0) {
+            integral_block_recurrence<S, BG>(it, lb, consts, prow, tile, red);
+            return;
+        }
+    }
     const EvalConst& K = consts[it.slot];
     const int R = it.r_steps, n = it.n_conv;
     const long C = it.n_cols;
@@ -364,16 +443,6 @@ __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
 #pragma unroll
         for (int c = 0; c <= S; c++) acc[c] = cmp[c] = 0.0;
 
-        // Uniform distance grid: the streams by recurrence along each row.
-        const double h = MIXED ? it.grid_step : 0.0;
-        double Cs[S > 0 ? S : 1], G[S > 0 ? S : 1], Rr[S > 0 ? S : 1];
-#pragma unroll
-        for (int s = 0; s < S; s++) {
-            const double d = sight.st[s].p * h;
-            Cs[s] = exp2_neg(2.0 * d * d);
-            G[s] = Rr[s] = 0.0;
-        }
-
         const int rt = kTileD2 / n;   // r-steps per LDS tile (host guarantees n <= kTileD2)
         const double2* rq = reinterpret_cast<const double2*>(it.rq);
         for (int k0 = 0; k0 < R; k0 += rt) {
@@ -386,15 +455,10 @@ __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
                 double bg = 0.0;
                 double st[kMaxStreams] = { 0.0, 0.0, 0.0, 0.0 };
                 const double2* row = tile + kk * n;
-                if (MIXED && h > 0.0) {
-                    row_by_recurrence<S, BG>(row, n, h, k % kRestartRows == 0, sight, Cs, G, Rr, K, bg,
-                                             st);
-                } else {
 #pragma unroll UNROLL
-                    for (int i = 0; i < n; i++) {
-                        const double2 p = row[i];
-                        point_terms<S, BG, MIXED, MIXED>(p.x, p.y, sight, K, bg, st);
-                    }
+                for (int i = 0; i < n; i++) {
+                    const double2 p = row[i];
+                    point_terms<S, BG, MIXED, MIXED>(p.x, p.y, sight, K, bg, st);
                 }
                 const RStep& rs = it.rstep[k];
                 double f = rs.reff_xr_rp3 * rs.irv;
