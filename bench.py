@@ -97,15 +97,18 @@ def cpu_seconds_per_eval(p, stars, rows_per_region, n_star_sample):
 # Workloads
 # --------------------------------------------------------------------------
 
-def mixed_roofline(points: float, rec: float, n_streams: int, ms: float) -> dict:
+def mixed_roofline(points: float, rec: float, n_streams: int, ms: float, bg: float = None) -> dict:
     """Roofline of a mixed-precision integral launch, each evaluation priced
     by the method that ran it: the frozen per-point model (FP32 exp, against
     the FP32 vector peak) for exponentials per point, the same weights over
     the recurrence's operations (FP64, against the FP64 peak) on distance
-    grids.  frac is the time the launch would take at those peaks over its
-    measured time."""
+    grids, where the background is charged per background evaluation (bg,
+    every 1st, 2nd or 4th node).  frac is the time the launch would take at
+    those peaks over its measured time."""
+    bg_rec = rec if bg is None else bg - (points - rec)
     w_exp = (points - rec) * workmodel.flops_per_point(n_streams)
-    w_rec = rec * workmodel.flops_per_point_recurrence(n_streams)
+    w_rec = (rec * workmodel.flops_per_point_recurrence(n_streams)
+             - (rec - bg_rec) * workmodel.flops_per_point(0))
     t_peak = (w_exp / FP32_PEAK_TFLOPS + w_rec / FP64_PEAK_TFLOPS) / 1e12
     work = w_exp + w_rec
     return {"achieved": round(work / (ms * 1e-3) / 1e12, 3),
@@ -113,11 +116,14 @@ def mixed_roofline(points: float, rec: float, n_streams: int, ms: float) -> dict
             "unit": "TFLOP/s", "frac": round(t_peak / (ms * 1e-3), 4),
             "integrand_evaluations_per_launch": int(points),
             "of_which_by_recurrence": int(rec),
+            "background_evaluations": int(bg_rec + points - rec),
             "algorithmic_flop_eq_per_launch": work,
             "pricing": (f"{workmodel.flops_per_point(n_streams)} flop-eq per evaluation with an exp "
                         f"per stream at {FP32_PEAK_TFLOPS} TFLOP/s (FP32), "
                         f"{workmodel.flops_per_point_recurrence(n_streams)} by recurrence at "
-                        f"{FP64_PEAK_TFLOPS} (FP64); peak is their work-weighted mix")}
+                        f"{FP64_PEAK_TFLOPS} (FP64), of which the background's "
+                        f"{workmodel.flops_per_point(0)} per background evaluation; peak is their "
+                        f"work-weighted mix")}
 
 
 def build_batch():
@@ -141,7 +147,7 @@ def full_size_fp64(steps: int, warmup: int, precision=sep.PREC_FP64):
     vecs = synth.parameter_vectors(1, p, 4)
     for i in range(warmup):
         ctx.eval(vecs[i % 4], precision=precision)
-    lat, kern, star, points, rec = [], [], [], [], []
+    lat, kern, star, points, rec, bgp = [], [], [], [], [], []
     t0 = time.perf_counter()
     for i in range(steps):
         s0 = time.perf_counter()
@@ -152,6 +158,7 @@ def full_size_fp64(steps: int, warmup: int, precision=sep.PREC_FP64):
         star.append(tm.star_ms)
         points.append(tm.integral_points)
         rec.append(tm.integral_points_recurrence)
+        bgp.append(tm.integral_points_background)
     el = time.perf_counter() - t0
     ctx.close()
     # Integrand evaluations of the integral pass: cells x n for the exact
@@ -165,6 +172,7 @@ def full_size_fp64(steps: int, warmup: int, precision=sep.PREC_FP64):
         "frac_fp64_peak": round(flops / (k_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
         "integrand_evaluations": int(np.mean(points)), "flop_eq": flops,
         "integrand_evaluations_by_recurrence": int(np.mean(rec)),
+        "background_evaluations": int(np.mean(bgp)),
         "resident_setup_s": round(setup, 3),
         "workload": "700x1600x640 + 700x400x160 cut (r x mu x nu), n=120, 3 streams, 200k stars",
     }
@@ -279,7 +287,8 @@ def bench_batch(args):
     log(f"batched mixed precision, headline ({args.batch_precision})")
     value, lat, elapsed, tm = timed_batch(pairs, vecs, head_prec, choice, args.steps, args.warmup)
     int_ms, points = tm.integral_ms, tm.integral_points
-    roof = mixed_roofline(float(points), float(tm.integral_points_recurrence), BATCH_STREAMS, int_ms)
+    roof = mixed_roofline(float(points), float(tm.integral_points_recurrence), BATCH_STREAMS, int_ms,
+                          float(tm.integral_points_background))
     log(f"  eager: persistent integral launch {int_ms:.2f} ms of {tm.total_ms:.2f} ms, "
         f"{points:.3e} integrand evaluations")
 
@@ -348,7 +357,7 @@ def bench_batch(args):
         v2, lat2, tm2, tune2 = second
         q2 = "exact" if quad == "resampled" else "resampled"
         roof2 = mixed_roofline(float(tm2.integral_points), float(tm2.integral_points_recurrence),
-                               BATCH_STREAMS, tm2.integral_ms)
+                               BATCH_STREAMS, tm2.integral_ms, float(tm2.integral_points_background))
         key2 = "mixed" if q2 == "exact" else "resampled"
         out[f"batched_mixed_{q2}_quadrature"] = {
             "value": round(v2, 4), "unit": "evals/s", "ratio_over_plain_fp64": round(v2 / plain, 3),
@@ -415,7 +424,8 @@ def bench_full(args):
                      dict({"bound": "valu", "kernel": "integral_kernel"},
                           **mixed_roofline(float(full["integrand_evaluations"]),
                                            float(full["integrand_evaluations_by_recurrence"]),
-                                           synth.FULL_SIZE_STREAMS, full["integral_kernel_ms"]),
+                                           synth.FULL_SIZE_STREAMS, full["integral_kernel_ms"],
+                                           float(full["background_evaluations"])),
                           traffic=None, avg_launch_ms=full["integral_kernel_ms"])),
     }
     if not args.no_cpu:

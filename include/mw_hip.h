@@ -129,6 +129,9 @@ typedef struct {
     long integral_points_recurrence;   /* of those, the ones on grids uniform in distance whose
                                           stream terms came from the FP64 recurrence instead of
                                           an exponential per point (resampled only) */
+    long integral_points_background;   /* background evaluations: integral_points, except on
+                                          distance grids whose background is wide enough to be
+                                          sampled on every 2nd or 4th node */
 } mw_timing;
 
 /* Flags for mw_ctx_create */

@@ -151,7 +151,7 @@ int run_plan(mw_ctx* c, Plan& P)
     struct Group { int bg; int first_item, n_items, first_block, n_blocks; };
     std::vector<Group> igroups;
     int total_iblocks = 0;
-    long points = 0, points_rec = 0;
+    long points = 0, points_rec = 0, points_bg = 0;
     if (P.integrals && n_pr > 0) {
         if ((rc = c->ritems.reserve(n_pr)) || (rc = c->h_ritems.reserve(n_pr)) ||
             (rc = c->rred.reserve(n_pr)) || (rc = c->h_rred.reserve(n_pr)) ||
@@ -172,7 +172,8 @@ int run_plan(mw_ctx* c, Plan& P)
             RegionDev& rd = *c->regions[pr.region];
             TableRef tr;
             const int level = select_level(rd, P.K[pr.slot], P.resample);
-            if ((rc = region_tables(rd, level, &tr))) return rc;
+            const int layout = level >= 0 ? table_layout(rd, level, P.K[pr.slot]) : 0;
+            if ((rc = region_tables(rd, level, layout, &tr))) return rc;
             RegionLaunch& L = c->h_ritems.p[item];
             L.rq = tr.rq;
             L.rstep = tr.rstep;
@@ -196,8 +197,11 @@ int run_plan(mw_ctx* c, Plan& P)
             L.block_begin = g.n_blocks;          // block index within the group's launch
             L.block_count = rd.blocks;
             L.n_streams = c->S;
-            L.grid_step = recurrence_step(level, P.K[pr.slot]);
+            L.grid_step = layout > 0 ? recurrence_step(level, P.K[pr.slot]) : 0.0;
+            L.grid_lo = tr.lo;
+            L.bg_stride = layout_bg_stride(layout);
             if (L.grid_step > 0.0) points_rec += rd.n_cols * tr.nodes;
+            points_bg += (level >= 0 ? rd.n_cols * tr.bg_nodes : (pr.end - pr.begin) * (long)rd.n_conv);
             ReduceItem& R = c->h_rred.p[item];
             R.block_begin = total_iblocks;       // global partial index
             R.block_count = rd.blocks;
@@ -335,6 +339,7 @@ int run_plan(mw_ctx* c, Plan& P)
     c->timing.integral_blocks = total_iblocks;
     c->timing.integral_points = points;
     c->timing.integral_points_recurrence = points_rec;
+    c->timing.integral_points_background = points_bg;
     c->timing.star_blocks = total_sblocks;
     return MW_OK;
 }
@@ -471,8 +476,9 @@ size_t mw_ctx_resident_bytes(const mw_ctx* c)
     for (const auto& r : c->regions) {
         b += r->rq.cap * sizeof(double) + r->rstep.cap * sizeof(RStep) + r->col.cap * sizeof(double);
         std::lock_guard<std::mutex> lock(r->level_mu);
-        for (const auto& q : r->levels)
-            if (q) b += q->rq.cap * sizeof(double) + q->rstep.cap * sizeof(RStep);
+        for (const auto& per_level : r->levels)
+            for (const auto& q : per_level)
+                if (q) b += q->rq.cap * sizeof(double) + q->rstep.cap * sizeof(RStep);
     }
     b += (c->star.cap + c->ratio.cap + c->kw.cap) * sizeof(double);
     return b;

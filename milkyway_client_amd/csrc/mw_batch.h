@@ -42,6 +42,7 @@ public:
     // Rewrites the staged constants of the planned slots (same modes).
     void set_constants(const std::vector<Slot>& slots);
     bool same_modes(const std::vector<Slot>& slots) const;
+    int table_choice(const RegionDev& rd, const EvalConst& k) const;
     // Enqueues the whole sequence on the runner's stream (capturable).  With
     // kernel_events, events bracket the integral launches (not while a
     // graph is being captured).
@@ -60,6 +61,7 @@ public:
     long integral_blocks() const { return total_iblocks_; }
     long integral_points() const { return points_; }
     long integral_points_recurrence() const { return points_rec_; }
+    long integral_points_background() const { return points_bg_; }
     long star_blocks() const { return total_sblocks_; }
 
     static size_t scratch_bytes_for(const std::vector<Slot>& slots);
@@ -74,9 +76,10 @@ private:
     hipEvent_t ev_[4] = {};
     bool mixed_ = false, resample_ = false;
     int variant_ = 0, blocks_per_cu_ = 2;
-    std::vector<int> levels_;        // quadrature level per (slot, region), slot-major
+    std::vector<int> levels_;        // quadrature table (level, layout) per (slot, region), slot-major
     long points_ = 0;                // integrand evaluations of the planned integral pass
     long points_rec_ = 0;            // of those, on distance grids by recurrence
+    long points_bg_ = 0;             // background evaluations (strided on distance grids)
     std::vector<int> slot_bg_, slot_n_regions_, slot_first_row_, slot_S_;
     std::vector<Group> igroups_, sgroups_;
     int n_ritems_ = 0, total_iblocks_ = 0, total_sblocks_ = 0;

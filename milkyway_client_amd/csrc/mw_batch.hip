@@ -35,15 +35,25 @@ int BatchRunner::init(int device, int flags)
     return MW_OK;
 }
 
+// Quadrature table of region rd under the model k, as one number: level and
+// layout.
+int BatchRunner::table_choice(const RegionDev& rd, const EvalConst& k) const
+{
+    const int level = select_level(rd, k, resample_);
+    return (level + 1) * kTableLayouts + (level >= 0 ? table_layout(rd, level, k) : 0);
+}
+
 bool BatchRunner::same_modes(const std::vector<Slot>& slots) const
 {
     if (slots.size() != slot_bg_.size()) return false;
     size_t li = 0;
     for (size_t s = 0; s < slots.size(); s++) {
         if (slots[s].k.bg_mode != slot_bg_[s]) return false;
-        // The planned work items point at one quadrature level per region.
+        // The planned work items point at one quadrature table per region and
+        // carry its layout (recurrence or not, background stride), all of
+        // which follow from the model.
         for (const auto& rd : slots[s].ctx->regions)
-            if (li >= levels_.size() || select_level(*rd, slots[s].k, resample_) != levels_[li++]) return false;
+            if (li >= levels_.size() || table_choice(*rd, slots[s].k) != levels_[li++]) return false;
     }
     return li == levels_.size();
 }
@@ -77,6 +87,7 @@ int BatchRunner::plan(const std::vector<Slot>& slots, int precision)
     levels_.clear();
     points_ = 0;
     points_rec_ = 0;
+    points_bg_ = 0;
     slot_bg_.assign(n, 0);
     slot_n_regions_.assign(n, 0);
     slot_first_row_.assign(n, 0);
@@ -101,9 +112,10 @@ int BatchRunner::plan(const std::vector<Slot>& slots, int precision)
         for (const auto& rd : c->regions) {
             TableRef tr;
             const int level = select_level(*rd, slots[s].k, resample_);
-            int rc0 = region_tables(*rd, level, &tr);
+            const int layout = level >= 0 ? table_layout(*rd, level, slots[s].k) : 0;
+            int rc0 = region_tables(*rd, level, layout, &tr);
             if (rc0) return rc0;
-            levels_.push_back(level);
+            levels_.push_back(table_choice(*rd, slots[s].k));
             points_ += rd->n_cols * tr.nodes;
             RegionLaunch L;
             std::memset(&L, 0, sizeof(L));
@@ -119,8 +131,11 @@ int BatchRunner::plan(const std::vector<Slot>& slots, int precision)
             L.out_index = row++;
             L.block_count = rd->blocks;
             L.n_streams = slots[s].k.n_streams;
-            L.grid_step = recurrence_step(level, slots[s].k);
+            L.grid_step = layout > 0 ? recurrence_step(level, slots[s].k) : 0.0;
+            L.grid_lo = tr.lo;
+            L.bg_stride = layout_bg_stride(layout);
             if (L.grid_step > 0.0) points_rec_ += rd->n_cols * tr.nodes;
+            points_bg_ += rd->n_cols * tr.bg_nodes;
             items.push_back(It{ L, (double)L.r_steps * L.n_conv * (2.0 + L.n_streams) });
         }
     }
@@ -474,6 +489,7 @@ int mw_batch_run(mw_batch* b, const double* params, mw_result* results)
     b->timing.integral_blocks = b->runner.integral_blocks();
     b->timing.integral_points = b->runner.integral_points();
     b->timing.integral_points_recurrence = b->runner.integral_points_recurrence();
+    b->timing.integral_points_background = b->runner.integral_points_background();
     b->timing.star_blocks = b->runner.star_blocks();
     return MW_OK;
 }

@@ -250,9 +250,15 @@ inline void reset_state(EVALUATION_STATE* es)
 // t = kQuadLevels + L: a grid in distance with spacing 2 * 2^(-L/2) kpc.
 constexpr int kQuadLevels = 13;
 constexpr int kQuadTables = 2 * kQuadLevels;
+// Layouts of a table: 0 = (r, weight) pairs for kernels that evaluate each
+// node's exponential; 1, 2, 3 = the recurrence layout on distance grids with
+// the background on every 1st, 2nd, 4th node.
+constexpr int kTableLayouts = 4;
+inline int layout_bg_stride(int layout) { return layout == 0 ? 0 : 1 << (layout - 1); }
 struct ResampledDev {
     int r_steps = 0, n_conv = 0;
     long nodes = 0;
+    double lo = 0.0;
     DevBuf<double> rq;
     DevBuf<RStep> rstep;
 };
@@ -272,7 +278,7 @@ struct RegionDev {
     std::vector<double> h_rq;
     std::vector<RStep> h_rstep;
     double r_max = 0.0, r_span = 0.0, g_span = 0.0, min_sin_gc = 1.0;
-    std::unique_ptr<ResampledDev> levels[kQuadTables];
+    std::unique_ptr<ResampledDev> levels[kQuadTables][kTableLayouts];
     std::mutex level_mu;
 };
 
@@ -283,6 +289,9 @@ struct TableRef {
     int r_steps, n_conv;
     int level;         // -1 for the exact tables
     long nodes;        // integrand evaluations per sight: r_steps x n_conv, or the grid's nodes
+    int layout;        // see kTableLayouts
+    double lo;         // resampled: the grid's first node
+    long bg_nodes;     // background evaluations per sight (nodes / stride in the recurrence layouts)
 };
 
 // Resampled table index to use for region rd under the model k, or -1 for
@@ -294,8 +303,13 @@ int select_level(const RegionDev& rd, const EvalConst& k, bool resample);
 // distance grid's spacing h when every stream's step p h <= k_s h stays
 // within 1/2, else 0 (evaluate each node's exponential).
 double recurrence_step(int level, const EvalConst& k);
-// The tables of that level, built and uploaded on first use.
-int region_tables(RegionDev& rd, int level, TableRef* out);
+// Layout of `level` to run under the model k (mixed kernels): 0 unless the
+// recurrence applies; then 1, 2 or 3 for the widest background stride
+// (1, 2, 4 nodes) whose spacing still meets the background's own bound
+// (0.15 b and 2 kpc).
+int table_layout(const RegionDev& rd, int level, const EvalConst& k);
+// The tables of that level and layout, built and uploaded on first use.
+int region_tables(RegionDev& rd, int level, int layout, TableRef* out);
 
 }  // namespace mw
 

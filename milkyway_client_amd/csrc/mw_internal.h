@@ -44,6 +44,7 @@ struct RegionTables {
     std::vector<double> rq;        // [r_steps][n_conv][2]: (r_point, qw_r3_N)
     std::vector<RStep> rstep;      // [r_steps]
     std::vector<double> col;       // [4][n_cols] SoA: cos b cos l, cos b sin l, sin b, ids
+    double lo = 0.0;               // resampled tables: the grid's first node
 };
 
 // Per-evaluation constants, one slot per parameter vector (device resident).
@@ -81,8 +82,11 @@ struct RegionLaunch {
     int out_index;             // index into the per-region result rows
     int block_begin, block_count;
     int n_streams;             // streams of the slot's model (batched kernels)
-    double grid_step;          // > 0: rows of nodes uniform in distance at this spacing
-                               // (mixed kernels take the stream Gaussians by recurrence)
+    double grid_step;          // > 0: rows of nodes uniform in distance at this spacing, in the
+                               // recurrence layout (mixed kernels take the stream Gaussians by
+                               // recurrence); node m of the region sits at grid_lo + m grid_step
+    double grid_lo;
+    int bg_stride;             // recurrence layout: background on every bg_stride-th node
 };
 
 // Stars resident on the device, SoA.
@@ -144,13 +148,22 @@ constexpr int kResampleRow = 32;      // grid nodes per pseudo r-step row (paddi
 // K over the grid g_lo + m delta, m < M; returns M (or -1 on bad input).
 long resample_quadrature(const double* g, const double* w, long n, double delta, int degree,
                          double* g_lo, std::vector<double>* K);
-// Tables in the layout of RegionTables (rq, rstep only) holding the
+// The same fold onto a given grid lo + m delta, m < M (the caller leaves
+// room for the stencils below and above the points).
+long resample_quadrature_at(const double* g, const double* w, long n, double delta, int degree,
+                            double lo, long M, std::vector<double>* K);
+// Tables in the layout of RegionTables (rq, rstep, lo) holding the
 // resampled quadrature of an exact region's rq / rstep on a grid of spacing
-// delta in r (in_r) or in g: pseudo r-steps of kResampleRow (r(x_m), K_m)
-// pairs with irv = reff = 1, and the aux-profile moments sum_k irv reff q_i
-// in the first row.  Returns M.
+// delta in r (in_r) or in g, in pseudo r-steps of kResampleRow nodes with
+// irv = reff = 1 and the aux-profile moments sum_k irv reff q_i in the
+// first row.  bg_stride 0 or 1: (r(x_m), K_m) pairs (1: from the origin a
+// background stride would use, which is the same).  bg_stride q = 2, 4 (in
+// r only; the kernels' strided recurrence layout): (K_m, Kb_m) pairs, K the
+// stream weights at spacing delta and Kb the background's at spacing
+// q delta on every q-th node (0 elsewhere), node m at r = lo + m delta.
+// Returns M.
 long build_resampled_tables(const std::vector<double>& rq, const std::vector<RStep>& rstep, int n_conv,
-                            double delta, bool in_r, RegionTables* out);
+                            double delta, bool in_r, int bg_stride, RegionTables* out);
 
 // Star-pass constants shared by every star of a context: r_i / r_star and
 // W_i * coeff * exp(-dx_i^2 / (2 sigma^2)).

@@ -296,14 +296,14 @@ __device__ __forceinline__ void point_terms(double r, double qw, const Sight<S>&
 // the next restart (the terms it stands for are below 2^-(40 - 15.5)^2).
 constexpr int kRestartRows = 8;
 
-template <int S, int BG>
-__device__ __forceinline__ void row_by_recurrence(const double2* row, int n, double h, bool restart,
-                                                  const Sight<S>& v, const double (&C)[S > 0 ? S : 1],
+template <int S, int BG, int Q>
+__device__ __forceinline__ void row_by_recurrence(const double2* row, int n, double r0, double h,
+                                                  bool restart, const Sight<S>& v,
+                                                  const double (&C)[S > 0 ? S : 1],
                                                   double (&G)[S > 0 ? S : 1],
                                                   double (&Rr)[S > 0 ? S : 1], const EvalConst& K,
                                                   double (&acc)[S + 1])
 {
-    const double r0 = row[0].x;
 #pragma unroll
     for (int s = 0; s < S; s++) {
         const double t0 = fma(v.st[s].p, r0, -v.st[s].m);
@@ -314,28 +314,51 @@ __device__ __forceinline__ void row_by_recurrence(const double2* row, int n, dou
             Rr[s] = exp2_neg(fma(2.0 * tc, d, d * d));
         }
     }
+    if constexpr (Q == 1) {
+        // (r, weight) entries, one weight for the background and the streams
 #pragma unroll 2
-    for (int i = 0; i < n; i++) {
-        const double2 p = row[i];
-        bg_point<BG>(p.x, p.y, v.bg, K, acc[0]);
+        for (int i = 0; i < n; i++) {
+            const double2 e = row[i];
+            bg_point<BG>(e.x, e.y, v.bg, K, acc[0]);
 #pragma unroll
-        for (int s = 0; s < S; s++) {
-            acc[1 + s] = fma(p.y, G[s], acc[1 + s]);
-            G[s] *= Rr[s];
-            Rr[s] *= C[s];
+            for (int s = 0; s < S; s++) {
+                acc[1 + s] = fma(e.y, G[s], acc[1 + s]);
+                G[s] *= Rr[s];
+                Rr[s] *= C[s];
+            }
+        }
+    } else {
+        // (stream weight, background weight) entries; node i sits at r0 + i h
+        // and only every Q-th node carries a background weight
+        double rb = r0;
+        for (int i = 0; i < n; i += Q) {
+            const double2 e = row[i];
+            bg_point<BG>(rb, e.y, v.bg, K, acc[0]);
+            rb += Q * h;
+#pragma unroll
+            for (int j = 0; j < Q; j++) {
+                const double w = j == 0 ? e.x : row[i + j].x;
+#pragma unroll
+                for (int s = 0; s < S; s++) {
+                    acc[1 + s] = fma(w, G[s], acc[1 + s]);
+                    G[s] *= Rr[s];
+                    Rr[s] *= C[s];
+                }
+            }
         }
     }
 }
 
 // One logical block of a resampled region on a distance grid (grid_step > 0;
-// the region is evaluated whole): the streams by recurrence.  The resampled
+// the region is evaluated whole, from tables in the recurrence layout): the
+// streams by recurrence, the background on every Q-th node.  The resampled
 // tables carry every weight in the node entries (each row's irv and reff
 // are 1), so the column sums take the terms directly, plainly summed (a
 // column has a few hundred nodes, not the exact quadrature's tens of
 // thousands of points); each stream's 2^-e is taken from the column geometry
 // again at the end rather than held through the loop.  Both keep the
 // register count within the occupancy targets.
-template <int S, int BG>
+template <int S, int BG, int Q>
 __device__ __forceinline__ void integral_block_recurrence(const RegionLaunch& it, long lb,
                                                           const EvalConst* __restrict__ consts,
                                                           double* __restrict__ prow, double2* tile,
@@ -370,8 +393,9 @@ __device__ __forceinline__ void integral_block_recurrence(const RegionLaunch& it
             __syncthreads();
             for (int kk = 0; kk < kt; kk++) {
                 const int k = k0 + kk;
-                row_by_recurrence<S, BG>(tile + kk * n, n, h, k % kRestartRows == 0, sight, Cs, G, Rr,
-                                         K, acc);
+                const double r0 = fma((double)k * n, h, it.grid_lo);
+                row_by_recurrence<S, BG, Q>(tile + kk * n, n, r0, h, k % kRestartRows == 0, sight, Cs, G,
+                                            Rr, K, acc);
                 if constexpr (BG == kBgHernquistAux) {
                     const RStep& rs = it.rstep[k];
                     acc[0] += K.bg_a * rs.q2 + K.bg_b * rs.q1 + K.bg_c * rs.q0;
@@ -412,7 +436,9 @@ __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
 {
     if constexpr (MIXED) {
         if (it.grid_step > 0.0) {
-            integral_block_recurrence<S, BG>(it, lb, consts, prow, tile, red);
+            if (it.bg_stride == 4) integral_block_recurrence<S, BG, 4>(it, lb, consts, prow, tile, red);
+            else if (it.bg_stride == 2) integral_block_recurrence<S, BG, 2>(it, lb, consts, prow, tile, red);
+            else integral_block_recurrence<S, BG, 1>(it, lb, consts, prow, tile, red);
             return;
         }
     }

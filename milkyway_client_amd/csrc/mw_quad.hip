@@ -97,32 +97,50 @@ double recurrence_step(int level, const EvalConst& k)
     return h;
 }
 
-int region_tables(RegionDev& rd, int level, TableRef* out)
+int table_layout(const RegionDev& rd, int level, const EvalConst& k)
+{
+    const double h = recurrence_step(level, k);
+    if (!(h > 0.0)) return 0;
+    const double b = kSunR0 * rd.min_sin_gc * std::min(1.0, 1.0 / k.q);
+    const double bg_delta = std::min(0.15 * b, kDeltaKpc);
+    if (4.0 * h <= bg_delta) return 3;
+    if (2.0 * h <= bg_delta) return 2;
+    return 1;
+}
+
+int region_tables(RegionDev& rd, int level, int layout, TableRef* out)
 {
     if (level < 0) {
-        *out = TableRef{ rd.rq.p, rd.rstep.p, rd.geom.r_steps, rd.n_conv, -1,
-                         (long)rd.geom.r_steps * rd.n_conv };
+        const long pts = (long)rd.geom.r_steps * rd.n_conv;
+        *out = TableRef{ rd.rq.p, rd.rstep.p, rd.geom.r_steps, rd.n_conv, -1, pts, 0, 0.0, pts };
         return MW_OK;
     }
-    if (level >= kQuadTables) return fail(MW_E_INVALID, "no such quadrature table");
+    if (level >= kQuadTables || layout < 0 || layout >= kTableLayouts || (layout > 0 && level < kQuadLevels))
+        return fail(MW_E_INVALID, "no such quadrature table");
     std::lock_guard<std::mutex> lock(rd.level_mu);
-    if (!rd.levels[level]) {
+    // Layout 1 (stride 1) is the (r, weight) table of layout 0.
+    std::unique_ptr<ResampledDev>& slot = rd.levels[level][layout == 1 ? 0 : layout];
+    if (!slot) {
         RegionTables t;
         const long M = build_resampled_tables(rd.h_rq, rd.h_rstep, rd.n_conv, table_delta(level),
-                                              level >= kQuadLevels, &t);
+                                              level >= kQuadLevels, layout == 1 ? 0 : layout_bg_stride(layout),
+                                              &t);
         if (M < 1) return fail(MW_E_INVALID, "resampled quadrature failed");
         auto q = std::make_unique<ResampledDev>();
         q->r_steps = t.r_steps;
         q->n_conv = t.n_conv;
         q->nodes = M;
+        q->lo = t.lo;
         int rc;
         if ((rc = q->rq.reserve(t.rq.size())) || (rc = q->rstep.reserve(t.rstep.size()))) return rc;
         MW_HIP_CHECK(hipMemcpy(q->rq.p, t.rq.data(), sizeof(double) * t.rq.size(), hipMemcpyHostToDevice));
         MW_HIP_CHECK(hipMemcpy(q->rstep.p, t.rstep.data(), sizeof(RStep) * t.rstep.size(), hipMemcpyHostToDevice));
-        rd.levels[level] = std::move(q);
+        slot = std::move(q);
     }
-    const ResampledDev& q = *rd.levels[level];
-    *out = TableRef{ q.rq.p, q.rstep.p, q.r_steps, q.n_conv, level, q.nodes };
+    const ResampledDev& q = *slot;
+    const int stride = layout_bg_stride(layout);
+    *out = TableRef{ q.rq.p, q.rstep.p, q.r_steps, q.n_conv, level, q.nodes, layout, q.lo,
+                     stride > 1 ? (q.nodes + stride - 1) / stride : q.nodes };
     return MW_OK;
 }
 

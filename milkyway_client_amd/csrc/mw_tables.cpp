@@ -472,18 +472,10 @@ void star_conv_tables(int n, std::vector<double>* ratio, std::vector<double>* kw
     }
 }
 
-long resample_quadrature(const double* g, const double* w, long n, double delta, int degree,
-                         double* g_lo, std::vector<double>* K)
+long resample_quadrature_at(const double* g, const double* w, long n, double delta, int degree,
+                            double lo, long M, std::vector<double>* K)
 {
-    if (n < 1 || !(delta > 0.0) || degree < 1 || degree > 31) return -1;
-    double gmin = g[0], gmax = g[0];
-    for (long i = 1; i < n; i++) {
-        gmin = std::min(gmin, g[i]);
-        gmax = std::max(gmax, g[i]);
-    }
-    const int half = degree / 2;
-    const double lo = gmin - (half + 1) * delta;
-    const long M = (long)std::ceil((gmax - lo) / delta) + half + 2;
+    if (n < 1 || !(delta > 0.0) || degree < 1 || degree > 31 || M < degree + 1) return -1;
     // Barycentric weights of degree + 1 equally spaced nodes: (-1)^i C(D, i).
     std::vector<long double> bw(degree + 1);
     long double c = 1.0L;
@@ -517,15 +509,31 @@ long resample_quadrature(const double* g, const double* w, long n, double delta,
     }
     K->assign(M, 0.0);
     for (long m = 0; m < M; m++) (*K)[m] = (double)acc[m];
-    *g_lo = lo;
     return M;
 }
 
+long resample_quadrature(const double* g, const double* w, long n, double delta, int degree,
+                         double* g_lo, std::vector<double>* K)
+{
+    if (n < 1 || !(delta > 0.0) || degree < 1 || degree > 31) return -1;
+    double gmin = g[0], gmax = g[0];
+    for (long i = 1; i < n; i++) {
+        gmin = std::min(gmin, g[i]);
+        gmax = std::max(gmax, g[i]);
+    }
+    const int half = degree / 2;
+    const double lo = gmin - (half + 1) * delta;
+    const long M = (long)std::ceil((gmax - lo) / delta) + half + 2;
+    *g_lo = lo;
+    return resample_quadrature_at(g, w, n, delta, degree, lo, M, K);
+}
+
 long build_resampled_tables(const std::vector<double>& rq, const std::vector<RStep>& rstep, int n_conv,
-                            double delta, bool in_r, RegionTables* out)
+                            double delta, bool in_r, int bg_stride, RegionTables* out)
 {
     const long R = (long)rstep.size();
     const long N = R * n_conv;
+    if (N < 1 || bg_stride < 0 || bg_stride > 4 || (bg_stride > 0 && !in_r)) return -1;
     std::vector<double> g(N), w(N);
     double Q0 = 0, Q1 = 0, Q2 = 0;
     for (long k = 0; k < R; k++) {
@@ -540,23 +548,55 @@ long build_resampled_tables(const std::vector<double>& rq, const std::vector<RSt
         }
     }
     double lo;
-    std::vector<double> K;
-    const long M = resample_quadrature(g.data(), w.data(), N, delta, kResampleDegree, &lo, &K);
-    if (M < 1) return -1;
+    long M;
+    std::vector<double> K, Kb;
+    if (bg_stride == 0) {
+        M = resample_quadrature(g.data(), w.data(), N, delta, kResampleDegree, &lo, &K);
+        if (M < 1) return -1;
+    } else {
+        // Two folds on one lattice: the streams at spacing delta, the
+        // background at q delta, from a common origin that leaves room for
+        // the coarser stencils, so background node j is stream node q j.
+        const int q = bg_stride;
+        const int half = kResampleDegree / 2;
+        const double gmin = *std::min_element(g.begin(), g.end());
+        const double gmax = *std::max_element(g.begin(), g.end());
+        lo = gmin - (half + 1) * q * delta;
+        const long Mb = (long)std::ceil((gmax - lo) / (q * delta)) + half + 2;
+        M = std::max((long)std::ceil((gmax - lo) / delta) + half + 2, (Mb - 1) * q + 1);
+        if (resample_quadrature_at(g.data(), w.data(), N, delta, kResampleDegree, lo, M, &K) < 1) return -1;
+        if (q == 1) {
+            Kb = K;
+        } else if (resample_quadrature_at(g.data(), w.data(), N, q * delta, kResampleDegree, lo, Mb, &Kb) < 1) {
+            return -1;
+        }
+    }
     const int row = kResampleRow;
     const long Rp = (M + row - 1) / row;
     out->r_steps = (int)Rp;
     out->n_conv = row;
+    out->lo = lo;
     out->rq.assign((size_t)2 * Rp * row, 0.0);
-    // Node distances: the grid itself in r, or r(g) = 10^((g - 4.2) / 5 + 1) / 1000.
-    auto node_r = [&](long m) {
-        const double x = lo + m * delta;
-        return in_r ? x : std::pow(10.0, (x - kAbsM) / 5.0 + 1.0) / 1000.0;
-    };
-    for (long m = 0; m < Rp * row; m++) {
-        // padding entries repeat the last node and carry no weight
-        out->rq[2 * m] = node_r(m < M ? m : M - 1);
-        out->rq[2 * m + 1] = m < M ? K[m] : 0.0;
+    if (bg_stride <= 1) {
+        // Node distances: the grid itself in r, or r(g) = 10^((g - 4.2) / 5 + 1) / 1000.
+        auto node_r = [&](long m) {
+            const double x = lo + m * delta;
+            return in_r ? x : std::pow(10.0, (x - kAbsM) / 5.0 + 1.0) / 1000.0;
+        };
+        for (long m = 0; m < Rp * row; m++) {
+            // padding entries repeat the last node and carry no weight
+            out->rq[2 * m] = node_r(m < M ? m : M - 1);
+            out->rq[2 * m + 1] = m < M ? K[m] : 0.0;
+        }
+    } else {
+        // (stream weight, background weight) per node; node m sits at
+        // r = lo + m delta, which the kernel computes.
+        const int q = bg_stride;
+        for (long m = 0; m < Rp * row; m++) {
+            out->rq[2 * m] = m < M ? K[m] : 0.0;
+            const long j = m / q;
+            out->rq[2 * m + 1] = (m % q == 0 && j < (long)Kb.size()) ? Kb[j] : 0.0;
+        }
     }
     out->rstep.assign(Rp, RStep{});
     for (long k = 0; k < Rp; k++) {
