@@ -104,6 +104,63 @@ def test_distance_grid_recurrence():
         assert lnp <= LNP_TOL
 
 
+def background_stride_probe(ctx, vec):
+    """(integral evaluations, background evaluations) of one resampled
+    evaluation of vec."""
+    ctx.eval(vec, precision=sep.PREC_RESAMPLED)
+    tm = ctx.timing()
+    return tm.integral_points, tm.integral_points_background
+
+
+def test_background_stride():
+    """Where the background is wider than the streams it is sampled on every
+    2nd or 4th node of the distance grid: fewer background evaluations than
+    nodes, and the background integral still within the bar."""
+    p, stars = synth.workload(91 + 3, LONG_R, 3)
+    ph, sh = ParamsHolder(p), StarsHolder(stars)
+    with sep.Context(ph, sh, timing=True) as ctx:
+        pts, bgp = background_stride_probe(ctx, p.parameter_vector())
+        got = ctx.eval(p.parameter_vector(), precision=sep.PREC_RESAMPLED)[0]
+    ref = O.evaluate_full(ParamsHolder(p), StarsHolder(stars))
+    e_bg = rel(got.background_integral, ref.background_integral)
+    print(f"{bgp} background evaluations for {pts} nodes; I_bg {e_bg:.2e}")
+    assert bgp <= 0.55 * pts
+    assert e_bg <= INT_REL_TOL
+
+
+def test_graph_batch_replans_when_the_layout_changes():
+    """A graph-replayed batch given vectors whose tables differ only in the
+    background stride (same grid level, different q) must re-plan: each
+    run equals the single evaluations of its own vectors."""
+    p, stars = synth.workload(131, LONG_R, 2)
+    for st in p.streams:      # narrow streams: a fine grid, the background free to stride
+        st.parameters[4] = 1.0
+    ph, sh = ParamsHolder(p), StarsHolder(stars)
+    base = p.parameter_vector()
+    with sep.Context(ph, sh, timing=True) as ctx:
+        seen = {}
+        for q in (0.45, 0.6, 0.8, 1.2, 1.6, 2.2, 3.0, 4.0):
+            v = base.copy()
+            v[0] = q
+            seen[q] = (v, background_stride_probe(ctx, v))
+        # same level (node counts within the stride tables' few extra nodes;
+        # levels differ by sqrt 2), different background stride
+        def stride(q):
+            return round(seen[q][1][0] / seen[q][1][1])
+        pairs = [(a, b) for a in seen for b in seen
+                 if abs(seen[a][1][0] - seen[b][1][0]) <= 0.1 * seen[a][1][0] and stride(a) != stride(b)]
+        assert pairs, f"no two q values share a level with different strides: {seen}"
+        qa, qb = pairs[0]
+        va, vb = seen[qa][0], seen[qb][0]
+        print(f"q {qa} vs {qb}: {seen[qa][1]} / {seen[qb][1]} (nodes, background evaluations)")
+        want_a = ctx.eval(va, precision=sep.PREC_RESAMPLED)[0]
+        want_b = ctx.eval(vb, precision=sep.PREC_RESAMPLED)[0]
+        with sep.Batch([(ctx, 1)], precision=sep.PREC_RESAMPLED, graph=True) as b:
+            for vec, want in ((va, want_a), (vb, want_b), (va, want_a)):
+                got = b.run([np.asarray([vec])])[0]
+                assert same(got, want)
+
+
 def test_aux_profile_fixture():
     p, stars = fixture("86", n_stars=1500, aux=True)
     e, _ = errors_vs_oracle(p, stars)
