@@ -304,9 +304,10 @@ int select_level(const RegionDev& rd, const EvalConst& k, bool resample);
 // within 1/2, else 0 (evaluate each node's exponential).
 double recurrence_step(int level, const EvalConst& k);
 // Layout of `level` to run under the model k (mixed kernels): 0 unless the
-// recurrence applies; then 1, 2 or 3 for the widest background stride
-// (1, 2, 4 nodes) whose spacing still meets the background's own bound
-// (0.15 b and 2 kpc).
+// recurrence applies (the Hernquist profile, whose mixed kernels carry it,
+// and a distance grid fine enough for every stream); then 1, 2 or 3 for the
+// widest background stride (1, 2, 4 nodes) whose spacing still meets the
+// background's own bound (0.15 b and 2 kpc).
 int table_layout(const RegionDev& rd, int level, const EvalConst& k);
 // The tables of that level and layout, built and uploaded on first use.
 int region_tables(RegionDev& rd, int level, int layout, TableRef* out);

@@ -99,6 +99,10 @@ double recurrence_step(int level, const EvalConst& k)
 
 int table_layout(const RegionDev& rd, int level, const EvalConst& k)
 {
+    // The recurrence lives in the mixed kernels, which exist for the
+    // Hernquist profile only; the other profiles run the FP64 kernels, which
+    // read (r, weight) tables.
+    if (k.bg_mode != kBgHernquist) return 0;
     const double h = recurrence_step(level, k);
     if (!(h > 0.0)) return 0;
     const double b = kSunR0 * rd.min_sin_gc * std::min(1.0, 1.0 / k.q);

@@ -161,6 +161,47 @@ def test_graph_batch_replans_when_the_layout_changes():
                 assert same(got, want)
 
 
+def aux_workload(seed, streams):
+    """A wedge large enough to be resampled on a distance grid, with narrow
+    streams (a fine grid) and the aux background profile (a, b, c = 100,
+    50, 25, as the reference's aux fixtures)."""
+    p, stars = synth.workload(seed, LONG_R, streams)
+    for st in p.streams:
+        st.parameters[4] = 1.0
+    p.aux_bg_profile = 1
+    p.background_parameters = list(p.background_parameters) + [100.0, 50.0, 25.0]
+    p.background_step = list(p.background_step) + [1.0, 1.0, 1.0]
+    p.background_min = list(p.background_min) + [-1000.0, -1000.0, -1000.0]
+    p.background_max = list(p.background_max) + [1000.0, 1000.0, 1000.0]
+    p.background_optimize = list(p.background_optimize) + [1, 1, 1]
+    return p, stars
+
+
+def test_aux_profile_resampled_single_and_batched():
+    """The aux profile runs the FP64 kernels in every precision: resampled,
+    its tables must stay in the (r, weight) layout (no recurrence, no
+    background stride).  Single and batched evaluations agree bitwise and
+    match the oracle."""
+    p, stars = aux_workload(141, 2)
+    ph, sh = ParamsHolder(p), StarsHolder(stars)
+    vec = p.parameter_vector()
+    with sep.Context(ph, sh, timing=True) as ctx:
+        single = ctx.eval(vec, precision=sep.PREC_RESAMPLED, per_star=True)[0]
+        tm = ctx.timing()
+        with sep.Batch([(ctx, 1)], precision=sep.PREC_RESAMPLED, graph=False) as b:
+            batched = b.run([np.asarray([vec])])[0]
+    ref = O.evaluate_full(ParamsHolder(p), StarsHolder(stars))
+    e = dict(I_bg=rel(single.background_integral, ref.background_integral),
+             I_st=rel(single.stream_integrals, ref.stream_integrals),
+             lnP=math.log(10) * np.max(np.abs(single.log10_p - ref.log10_p)))
+    print({k: f"{v:.2e}" for k, v in e.items()}, "points", tm.integral_points,
+          "recurrence", tm.integral_points_recurrence, "background", tm.integral_points_background)
+    assert tm.integral_points_recurrence == 0
+    assert tm.integral_points_background == tm.integral_points
+    assert same(batched, single)
+    assert e["I_bg"] <= INT_REL_TOL and e["I_st"] <= INT_REL_TOL and e["lnP"] <= LNP_TOL
+
+
 def test_aux_profile_fixture():
     p, stars = fixture("86", n_stars=1500, aux=True)
     e, _ = errors_vs_oracle(p, stars)
