@@ -24,12 +24,16 @@ from . import separation as sep
 DEFAULT_CACHE = Path(os.environ.get("MW_TUNING_CACHE",
                                     Path.home() / ".cache" / "mw_separation" / "tuning.json"))
 BLOCKS_PER_CU = (1, 2, 4, 8)
+# Bumped whenever the library's table of kernel variants changes meaning
+# (variant v = unroll {2, 4} x occupancy target {compiler's, 4, 5} waves),
+# so that cached choices of an older table are not reused.
+VARIANT_TABLE = 2
 
 
 def signature(groups, precision: int) -> str:
     """groups: list of (Context, n_vectors, shape-description tuple)."""
     parts = sorted(f"{desc}x{n}" for _, n, desc in groups)
-    return f"p{precision}:" + ";".join(parts)
+    return f"t{VARIANT_TABLE}p{precision}:" + ";".join(parts)
 
 
 def load_cache(path: Path = DEFAULT_CACHE) -> Dict:

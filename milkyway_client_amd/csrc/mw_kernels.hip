@@ -1000,8 +1000,8 @@ hipError_t launch_finalize(const int* d_first, const int* d_count, const double*
 // Batched (persistent) launchers and their tunable variants
 // ---------------------------------------------------------------------------
 // Variant v: inner-loop unroll factor and waves-per-EU occupancy target.
-constexpr int kVariantUnroll[] = { 2, 1, 4, 2, 1, 4 };
-constexpr int kVariantWaves[] = { 1, 1, 1, 5, 5, 5 };
+constexpr int kVariantUnroll[] = { 2, 4, 2, 4, 2, 4 };
+constexpr int kVariantWaves[] = { 1, 1, 4, 4, 5, 5 };
 constexpr int kNumVariants = 6;
 
 using PersistentFn = void (*)(const RegionLaunch*, int, int, const EvalConst*, double*, unsigned*);
