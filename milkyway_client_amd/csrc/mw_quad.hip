@@ -11,12 +11,14 @@
 // A grid uniform in distance resolves the streams with the same spacing at
 // every distance; a grid in magnitude is finer near the Sun, which is where a
 // sight passing close to the Galactic centre needs it.  Both are sized, the
-// one with fewer nodes is used.  With 16-node stencils at spacing 0.25 sigma
+// one with fewer nodes is used.  With 16-node stencils at spacing 0.3 sigma
 // (distance) or 0.3 w (magnitude) for streams and 0.15 b / 0.2 w for the
-// background, the per-sight quadrature error stays near 1e-10 relative
-// (measured on a full-size point set: 1 kpc streams crossing out to 100 kpc,
-// 8e-10 at 0.3 sigma and 9e-13 at 0.2 sigma in distance, 3e-13 at 0.005 mag
-// in magnitude; a sight 0.14 kpc from the centre, 1e-12 at 0.005 mag).
+// background, the per-sight quadrature error stays below 1e-9 relative for a
+// stream crossing the sight at right angles, the narrowest case (measured on
+// a full-size point set: 1 kpc streams crossing out to 100 kpc, 8e-10 at
+// 0.3 sigma and 9e-13 at 0.2 sigma in distance, 3e-13 at 0.005 mag in
+// magnitude; a sight 0.14 kpc from the centre, 1e-12 at 0.005 mag).  Levels
+// are spaced by sqrt 2, so the spacing used is 0.21-0.3 sigma.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -68,7 +70,7 @@ int select_level(const RegionDev& rd, const EvalConst& k, bool resample)
     const double dg = std::min(0.3 * kMagPerLn * sigma / rd.r_max, 0.2 * kMagPerLn * b / kSunR0);
     const int Lg = level_for(dg, kDeltaMag);
     // grid in distance
-    const double dr = std::min(0.25 * sigma, 0.15 * b);
+    const double dr = std::min(0.3 * sigma, 0.15 * b);
     const int Lr = level_for(dr, kDeltaKpc);
     long best = -1;
     int t = -1;

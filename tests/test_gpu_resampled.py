@@ -117,6 +117,8 @@ def test_background_stride():
     2nd or 4th node of the distance grid: fewer background evaluations than
     nodes, and the background integral still within the bar."""
     p, stars = synth.workload(91 + 3, LONG_R, 3)
+    for st in p.streams:      # narrow streams: a fine grid, the background free to stride
+        st.parameters[4] = 1.0
     ph, sh = ParamsHolder(p), StarsHolder(stars)
     with sep.Context(ph, sh, timing=True) as ctx:
         pts, bgp = background_stride_probe(ctx, p.parameter_vector())
