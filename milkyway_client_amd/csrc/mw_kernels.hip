@@ -296,56 +296,73 @@ __device__ __forceinline__ void point_terms(double r, double qw, const Sight<S>&
 // the next restart (the terms it stands for are below 2^-(40 - 15.5)^2).
 constexpr int kRestartRows = 8;
 
+// A stream's terms in a row are skipped, by the whole wave, when every one of
+// them on every lane lies below 2^-kNegligibleLog2 of that sight's largest
+// term over the region's nodes (2^-(min t^2)): what is dropped is then below
+// 2^-70 relative, per sight, whatever the weights.
+constexpr double kNegligibleLog2 = 70.0;
+
+// Smallest t^2 over the nodes from t_first to t_last (t rises with r): 0
+// when the Gaussian's peak lies between them.
+__device__ __forceinline__ double min_t2(double t_first, double t_last)
+{
+    if (t_first <= 0.0 && t_last >= 0.0) return 0.0;
+    const double t = fmin(fabs(t_first), fabs(t_last));
+    return t * t;
+}
+
 template <int S, int BG, int Q>
 __device__ __forceinline__ void row_by_recurrence(const double2* row, int n, double r0, double h,
                                                   bool restart, const Sight<S>& v,
                                                   const double (&C)[S > 0 ? S : 1],
+                                                  const float (&floor_t2)[S > 0 ? S : 1],
                                                   double (&G)[S > 0 ? S : 1],
                                                   double (&Rr)[S > 0 ? S : 1], const EvalConst& K,
                                                   double (&acc)[S + 1])
 {
-#pragma unroll
-    for (int s = 0; s < S; s++) {
-        const double t0 = fma(v.st[s].p, r0, -v.st[s].m);
-        if (restart || (t0 < 0.0 && G[s] < 0x1p-1000)) {
-            const double tc = fmax(t0, -40.0);
-            const double d = v.st[s].p * h;
-            G[s] = exp2_neg(tc * tc);
-            Rr[s] = exp2_neg(fma(2.0 * tc, d, d * d));
-        }
-    }
     if constexpr (Q == 1) {
         // (r, weight) entries, one weight for the background and the streams
 #pragma unroll 2
         for (int i = 0; i < n; i++) {
             const double2 e = row[i];
             bg_point<BG>(e.x, e.y, v.bg, K, acc[0]);
-#pragma unroll
-            for (int s = 0; s < S; s++) {
-                acc[1 + s] = fma(e.y, G[s], acc[1 + s]);
-                G[s] *= Rr[s];
-                Rr[s] *= C[s];
-            }
         }
     } else {
         // (stream weight, background weight) entries; node i sits at r0 + i h
         // and only every Q-th node carries a background weight
         double rb = r0;
         for (int i = 0; i < n; i += Q) {
-            const double2 e = row[i];
-            bg_point<BG>(rb, e.y, v.bg, K, acc[0]);
+            bg_point<BG>(rb, row[i].y, v.bg, K, acc[0]);
             rb += Q * h;
-#pragma unroll
-            for (int j = 0; j < Q; j++) {
-                const double w = j == 0 ? e.x : row[i + j].x;
-#pragma unroll
-                for (int s = 0; s < S; s++) {
-                    acc[1 + s] = fma(w, G[s], acc[1 + s]);
-                    G[s] *= Rr[s];
-                    Rr[s] *= C[s];
-                }
-            }
         }
+    }
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const double p = v.st[s].p;
+        const double d = p * h;
+        const double t0 = fma(p, r0, -v.st[s].m);
+        const double t1 = fma(d, (double)(n - 1), t0);
+        if (__all(min_t2(t0, t1) > (double)floor_t2[s])) {
+            // Negligible here; a later row before the peak restarts from G = 0.
+            G[s] = 0.0;
+            continue;
+        }
+        if (restart || (t0 < 0.0 && G[s] < 0x1p-1000)) {
+            const double tc = fmax(t0, -40.0);
+            G[s] = exp2_neg(tc * tc);
+            Rr[s] = exp2_neg(fma(2.0 * tc, d, d * d));
+        }
+        double g = G[s], rr = Rr[s], a = acc[1 + s];
+        const double c = C[s];
+#pragma unroll 4
+        for (int i = 0; i < n; i++) {
+            a = fma(Q == 1 ? row[i].y : row[i].x, g, a);
+            g *= rr;
+            rr *= c;
+        }
+        G[s] = g;
+        Rr[s] = rr;
+        acc[1 + s] = a;
     }
 }
 
@@ -378,11 +395,16 @@ __device__ __forceinline__ void integral_block_recurrence(const RegionLaunch& it
     {
         const Sight<S> sight = make_sight<S, BG>(it.col[cc], it.col[C + cc], it.col[2 * C + cc], K);
         double Cs[S > 0 ? S : 1], G[S > 0 ? S : 1], Rr[S > 0 ? S : 1];
+        float floor_t2[S > 0 ? S : 1];   // rows whose t^2 all exceed this are negligible
+        const double r_last = fma((double)R * n - 1.0, h, it.grid_lo);
 #pragma unroll
         for (int s = 0; s < S; s++) {
             const double d = sight.st[s].p * h;
             Cs[s] = exp2_neg(2.0 * d * d);
             G[s] = Rr[s] = 0.0;
+            const double lo = min_t2(fma(sight.st[s].p, it.grid_lo, -sight.st[s].m),
+                                     fma(sight.st[s].p, r_last, -sight.st[s].m));
+            floor_t2[s] = (float)(lo + kNegligibleLog2);   // rounding either way moves the bound by < 1e-5
         }
         const int rt = kTileD2 / n;
         const double2* rq = reinterpret_cast<const double2*>(it.rq);
@@ -394,8 +416,8 @@ __device__ __forceinline__ void integral_block_recurrence(const RegionLaunch& it
             for (int kk = 0; kk < kt; kk++) {
                 const int k = k0 + kk;
                 const double r0 = fma((double)k * n, h, it.grid_lo);
-                row_by_recurrence<S, BG, Q>(tile + kk * n, n, r0, h, k % kRestartRows == 0, sight, Cs, G,
-                                            Rr, K, acc);
+                row_by_recurrence<S, BG, Q>(tile + kk * n, n, r0, h, k % kRestartRows == 0, sight, Cs,
+                                            floor_t2, G, Rr, K, acc);
                 if constexpr (BG == kBgHernquistAux) {
                     const RStep& rs = it.rstep[k];
                     acc[0] += K.bg_a * rs.q2 + K.bg_b * rs.q1 + K.bg_c * rs.q0;
