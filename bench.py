@@ -318,7 +318,8 @@ def bench_batch(args):
     quad_desc = ("resampled: each region's r-step x convolution weights folded by 16-node Lagrange "
                  "interpolation onto a grid in distance or magnitude sized to the model's narrowest "
                  "feature (MW_PREC_RESAMPLED); on distance grids the stream Gaussians by FP64 "
-                 "recurrence; star pass exact" if quad == "resampled" else
+                 "recurrence (rows a stream does not reach skipped) and the background on every 1st, "
+                 "2nd or 4th node as its width allows; star pass exact" if quad == "resampled" else
                  "exact: the reference's r-step x convolution points (MW_PREC_MIXED)")
     out = {
         "metric": "likelihood evals/s, batched mixed precision vs plain FP64 @ per-star |d lnP| <= 1e-7",
