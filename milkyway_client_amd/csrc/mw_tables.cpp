@@ -472,43 +472,81 @@ void star_conv_tables(int n, std::vector<double>* ratio, std::vector<double>* kw
     }
 }
 
-long resample_quadrature_at(const double* g, const double* w, long n, double delta, int degree,
-                            double lo, long M, std::vector<double>* K)
+namespace {
+
+// Fold points [p0, p1) into acc (one per grid node).  The stencil is
+// computed in double (relative error ~1e-16 per weight) and accumulated in
+// long double.
+void fold_points(const double* g, const double* w, long p0, long p1, double delta, int degree,
+                 double lo, long M, const std::vector<double>& bw, std::vector<long double>* acc_out)
 {
-    if (n < 1 || !(delta > 0.0) || degree < 1 || degree > 31 || M < degree + 1) return -1;
-    // Barycentric weights of degree + 1 equally spaced nodes: (-1)^i C(D, i).
-    std::vector<long double> bw(degree + 1);
-    long double c = 1.0L;
-    for (int i = 0; i <= degree; i++) {
-        bw[i] = (i % 2 ? -c : c);
-        c = c * (degree - i) / (i + 1);
-    }
-    std::vector<long double> acc(M, 0.0L), lw(degree + 1);
-    for (long p = 0; p < n; p++) {
+    std::vector<long double>& acc = *acc_out;
+    double lw[32];
+    for (long p = p0; p < p1; p++) {
         // Stencil of degree + 1 nodes around g[p], centred on its interval.
-        const long double x = ((long double)g[p] - lo) / delta;
-        long i0 = (long)std::floor((double)x) - (degree - 1) / 2;
+        const double x = (g[p] - lo) / delta;
+        long i0 = (long)std::floor(x) - (degree - 1) / 2;
         if (i0 < 0) i0 = 0;
         if (i0 > M - degree - 1) i0 = M - degree - 1;
+        const double u = x - (double)i0;
         int hit = -1;
-        long double den = 0.0L;
-        for (int i = 0; i <= degree; i++) {
-            const long double t = x - (long double)(i0 + i);
-            if (t == 0.0L) {
-                hit = i;
-                break;
-            }
-            lw[i] = bw[i] / t;
-            den += lw[i];
-        }
+        for (int i = 0; i <= degree; i++)
+            if (u == (double)i) hit = i;
         if (hit >= 0) {
             acc[i0 + hit] += w[p];
             continue;
         }
-        for (int i = 0; i <= degree; i++) acc[i0 + i] += (long double)w[p] * (lw[i] / den);
+        double den = 0.0;
+        for (int i = 0; i <= degree; i++) {
+            lw[i] = bw[i] / (u - (double)i);
+            den += lw[i];
+        }
+        const double scale = w[p] / den;
+        for (int i = 0; i <= degree; i++) acc[i0 + i] += (long double)(lw[i] * scale);
+    }
+}
+
+}  // namespace
+
+long resample_quadrature_at(const double* g, const double* w, long n, double delta, int degree,
+                            double lo, long M, std::vector<double>* K)
+{
+    if (n < 1 || !(delta > 0.0) || degree < 1 || degree > 31 || M < degree + 1) return -1;
+    // Barycentric weights of degree + 1 equally spaced nodes: (-1)^i C(D, i)
+    // (exact in double up to degree 31).
+    std::vector<double> bw(degree + 1);
+    double c = 1.0;
+    for (int i = 0; i <= degree; i++) {
+        bw[i] = (i % 2 ? -c : c);
+        c = c * (degree - i) / (i + 1);
+    }
+    // A full-size region has ~1e5 points: the fold runs over a fixed number
+    // of point chunks on up to that many threads, each chunk into its own
+    // accumulator, and the chunks are added in chunk order, so the result
+    // does not depend on the machine's thread count.  (A table is built the
+    // first time a model needs it, on the thread that plans the launch.)
+    const int chunks = n >= 4096 ? 16 : 1;
+    std::vector<std::vector<long double>> acc(chunks, std::vector<long double>(M, 0.0L));
+    auto run = [&](int ch) {
+        fold_points(g, w, n * ch / chunks, n * (ch + 1) / chunks, delta, degree, lo, M, bw, &acc[ch]);
+    };
+    const int threads = std::max(1, std::min<int>(chunks, (int)std::thread::hardware_concurrency()));
+    if (threads == 1) {
+        for (int ch = 0; ch < chunks; ch++) run(ch);
+    } else {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < threads; t++)
+            pool.emplace_back([&, t] {
+                for (int ch = t; ch < chunks; ch += threads) run(ch);
+            });
+        for (std::thread& th : pool) th.join();
     }
     K->assign(M, 0.0);
-    for (long m = 0; m < M; m++) (*K)[m] = (double)acc[m];
+    for (long m = 0; m < M; m++) {
+        long double sum = 0.0L;
+        for (int ch = 0; ch < chunks; ch++) sum += acc[ch][m];
+        (*K)[m] = (double)sum;
+    }
     return M;
 }
 
