@@ -48,12 +48,35 @@ def errors_vs_oracle(p, stars):
     return e, grew
 
 
-@pytest.mark.parametrize("S", [1, 2, 3, 4])
+@pytest.mark.parametrize("S", [0, 1, 2, 3, 4])
 def test_resampled_against_oracle(S):
     p, stars = synth.workload(90 + S, LONG_R, S)
     e, grew = errors_vs_oracle(p, stars)
     assert grew, "the resampled grid was not used"
     assert e["lnP"] <= LNP_TOL and e["lnL"] <= LNP_TOL
+    assert e["I_bg"] <= INT_REL_TOL and e["I_st"] <= INT_REL_TOL
+
+
+def test_disabled_stream_under_the_recurrence():
+    """A stream switched off by |sigma| < 1e-4 (evaluation_optimized.c:279-280)
+    carries k_s = 0 in the model constants, so its recurrence is the
+    constant 1 (no overflow) and the off flag zeroes it: the recurrence stays
+    on, the disabled stream's integral is exactly 0 and the others match the
+    oracle.  (The reference's star pass then divides 0 by the zero integral,
+    so only the integrals are compared.)"""
+    p, stars = synth.workload(133, LONG_R, 2)
+    p.streams[1].parameters[4] = 1e-5
+    ph, sh = ParamsHolder(p), StarsHolder(stars)
+    with sep.Context(ph, sh, timing=True) as ctx:
+        got = ctx.eval(p.parameter_vector(), precision=sep.PREC_RESAMPLED)[0]
+        tm = ctx.timing()
+    ref = O.integrals(ParamsHolder(p), StarsHolder(stars))
+    e = dict(I_bg=rel(got.background_integral, ref.es.background_integral),
+             I_st=rel(got.stream_integrals[0], ref.stream_integrals()[0]))
+    print({k: f"{v:.2e}" for k, v in e.items()}, "stream 2 integral", got.stream_integrals[1],
+          "recurrence", tm.integral_points_recurrence, "of", tm.integral_points)
+    assert got.stream_integrals[1] == 0.0
+    assert tm.integral_points_recurrence == tm.integral_points > 0
     assert e["I_bg"] <= INT_REL_TOL and e["I_st"] <= INT_REL_TOL
 
 
