@@ -83,7 +83,7 @@ int mw_ctx_hessian(mw_ctx* ctx, const double* point, const double* step, int pre
     HessianEval h{ ctx, precision };
     fail(MW_OK, "");
     const int rc = mw_hessian(ctx->n_params, point, step, eval_vectors, &h, hessian, gradient, value);
-    // Failures of the evaluations carry their own message; the driver's own
+    // Failures of the evaluations carry their own message; mw_hessian's own
     // failure can only be an allocation.
     if (rc && mw_last_error()[0] == '\0') return fail(MW_E_NOMEM, "Hessian stencil allocation failed");
     return rc;
