@@ -110,6 +110,8 @@ int build_context(mw_ctx* c, const std::vector<RegionGeom>& geoms, int n_conv, i
             const double ux = t.col[col];
             rd->min_sin_gc = std::min(rd->min_sin_gc, std::sqrt(std::max(0.0, 1.0 - ux * ux)));
         }
+        tabulate_bg_width(*rd, t.col.data(), t.col.data() + t.n_cols, t.col.data() + 2 * t.n_cols,
+                          g.mu_steps, g.nu_steps);
         rd->h_rq = std::move(t.rq);
         rd->h_rstep = std::move(t.rstep);
         c->regions.push_back(std::move(rd));

@@ -278,6 +278,10 @@ struct RegionDev {
     std::vector<double> h_rq;
     std::vector<RStep> h_rstep;
     double r_max = 0.0, r_span = 0.0, g_span = 0.0, min_sin_gc = 1.0;
+    // Smallest width of the background's peak along the region's sights
+    // (kpc), tabulated at q = 2^(j / 4 - kWidthQ0), from a grid of sights
+    // that includes the region's edges (see bg_width).
+    std::vector<float> bg_width_q;
     std::unique_ptr<ResampledDev> levels[kQuadTables][kTableLayouts];
     std::mutex level_mu;
 };
@@ -293,6 +297,15 @@ struct TableRef {
     double lo;         // resampled: the grid's first node
     long bg_nodes;     // background evaluations per sight (nodes / stride in the recurrence layouts)
 };
+
+// Tabulate RegionDev::bg_width_q from the region's sight directions
+// (ux, uy, uz arrays of n_cols over mu_steps x nu_steps, mu-major).
+constexpr int kWidthQ = 33, kWidthQ0 = 4;
+void tabulate_bg_width(RegionDev& rd, const double* ux, const double* uy, const double* uz, int mu_steps,
+                       int nu_steps);
+// A lower bound on the width of the background's peak along any sight of the
+// region for flattening q (kpc).
+double bg_width(const RegionDev& rd, double q);
 
 // Resampled table index to use for region rd under the model k, or -1 for
 // the exact tables (resampling off, a background mode it does not cover, or

@@ -5,9 +5,13 @@
 //   stream s: a Gaussian of width >= sigma_s along the sight (in distance),
 //             i.e. >= (5 / ln 10) sigma_s / r_max in magnitude;
 //   background: 1 / (rg (rg + r0)^3) peaks where the sight passes closest to
-//             the Galactic centre, at distance b >= 8.5 min sin(GC angle)
-//             min(1, 1/q) and r <= 8.5: a width of at least b in distance,
-//             (5 / ln 10) b / 8.5 in magnitude.
+//             the Galactic centre.  Along a sight rg^2 = P (r - r_c)^2 + rg_min^2
+//             with P = u_x^2 + u_y^2 + u_z^2 / q^2, so the peak is
+//             rg_min / sqrt(P) wide: rg_min >= d min(1, 1/q) for a sight that
+//             passes the centre at distance d = 8.5 sin(GC angle), and
+//             sqrt(P) <= max(1, 1/q), which gives a width of at least
+//             b = 8.5 min sin(GC angle) min(q, 1/q) in distance (r <= 8.5 at
+//             the peak) and (5 / ln 10) b / 8.5 in magnitude.
 // A grid uniform in distance resolves the streams with the same spacing at
 // every distance; a grid in magnitude is finer near the Sun, which is where a
 // sight passing close to the Galactic centre needs it.  Both are sized, the
@@ -56,6 +60,57 @@ long nodes_for(double span, double delta)
 
 }  // namespace
 
+// Width of the background's peak along one sight u (unit vector) for
+// flattening q: rg^2 = P (r - r_c)^2 + rg_min^2 with P = u_x^2 + u_y^2 +
+// u_z^2 / q^2, r_c = 8.5 u_x / P and rg_min^2 = 8.5^2 (u_y^2 + u_z^2 / q^2) / P,
+// so the peak is rg_min / sqrt(P) wide.  A sight pointing away from the
+// centre (u_x <= 0) has r_c <= 0: along r >= 0 it varies on the scale
+// rg(0) / sqrt(P) = 8.5 / sqrt(P).
+static double sight_bg_width(double ux, double uy, double uz, double q)
+{
+    const double side = uy * uy + uz * uz / (q * q);
+    const double P = ux * ux + side;
+    return ux > 0.0 ? kSunR0 * std::sqrt(side) / P : kSunR0 / std::sqrt(P);
+}
+
+void tabulate_bg_width(RegionDev& rd, const double* ux, const double* uy, const double* uz, int mu_steps,
+                       int nu_steps)
+{
+    // 64 x 64 sights, the first and last rows and columns included: the
+    // width varies by a few percent between neighbouring samples, which the
+    // margin in bg_width covers.
+    std::vector<long> rows, cols;
+    const int nm = std::min(mu_steps, 64), nn = std::min(nu_steps, 64);
+    for (int i = 0; i < nm; i++) rows.push_back(nm > 1 ? (long)i * (mu_steps - 1) / (nm - 1) : 0);
+    for (int j = 0; j < nn; j++) cols.push_back(nn > 1 ? (long)j * (nu_steps - 1) / (nn - 1) : 0);
+    rd.bg_width_q.assign(kWidthQ, INFINITY);
+    for (int t = 0; t < kWidthQ; t++) {
+        const double q = std::exp2(t / 4.0 - kWidthQ0);
+        double w = INFINITY;
+        for (long r : rows)
+            for (long c : cols) {
+                const long i = r * nu_steps + c;
+                w = std::min(w, sight_bg_width(ux[i], uy[i], uz[i], q));
+            }
+        rd.bg_width_q[t] = (float)w;
+    }
+}
+
+double bg_width(const RegionDev& rd, double q)
+{
+    // Below 0.85 of the smaller of the two tabulated q around it: the width
+    // changes by under 19 % from one tabulated q to the next and by a few
+    // percent between sampled sights.  Outside the table, the closed-form
+    // bound 8.5 min sin(GC angle) min(q, 1/q).
+    const double bound = kSunR0 * rd.min_sin_gc * std::min(q, 1.0 / q);
+    if (rd.bg_width_q.size() != (size_t)kWidthQ || !(q > 0.0)) return bound;
+    const double x = (std::log2(q) + kWidthQ0) * 4.0;
+    if (!(x >= 0.0) || !(x <= kWidthQ - 1)) return bound;
+    const int t = std::min((int)x, kWidthQ - 2);
+    const double w = 0.85 * std::min(rd.bg_width_q[t], rd.bg_width_q[t + 1]);
+    return std::max(w, bound);
+}
+
 int select_level(const RegionDev& rd, const EvalConst& k, bool resample)
 {
     if (!resample || rd.h_rq.empty()) return -1;
@@ -65,7 +120,7 @@ int select_level(const RegionDev& rd, const EvalConst& k, bool resample)
         if (k.on[s] == 0.0 || !(k.inv_2s2[s] > 0.0)) continue;
         sigma = std::min(sigma, std::sqrt(0.5 / k.inv_2s2[s]));
     }
-    const double b = kSunR0 * rd.min_sin_gc * std::min(1.0, 1.0 / k.q);
+    const double b = bg_width(rd, k.q);
     // grid in magnitude
     const double dg = std::min(0.3 * kMagPerLn * sigma / rd.r_max, 0.2 * kMagPerLn * b / kSunR0);
     const int Lg = level_for(dg, kDeltaMag);
@@ -107,7 +162,7 @@ int table_layout(const RegionDev& rd, int level, const EvalConst& k)
     if (k.bg_mode != kBgHernquist) return 0;
     const double h = recurrence_step(level, k);
     if (!(h > 0.0)) return 0;
-    const double b = kSunR0 * rd.min_sin_gc * std::min(1.0, 1.0 / k.q);
+    const double b = bg_width(rd, k.q);
     const double bg_delta = std::min(0.15 * b, kDeltaKpc);
     if (4.0 * h <= bg_delta) return 3;
     if (2.0 * h <= bg_delta) return 2;

@@ -233,6 +233,55 @@ def test_aux_profile_fixture():
     assert e["lnP"] <= LNP_TOL and e["I_bg"] <= INT_REL_TOL and e["I_st"] <= INT_REL_TOL
 
 
+def sweep_vectors(p, count, seed):
+    """Seeded vectors over the synthetic ranges, and beyond them at the
+    extremes: q from 0.3 to 1.0 and one above 1, r0 at 2 and 12 kpc, sigma
+    from 0.6 to 12 kpc, stream centres from 2 to 60 kpc, and streams switched
+    off (|sigma| < 1e-4)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    base = synth.parameter_vectors(seed % 1000, p, count)
+    S = p.number_streams
+    for j in range(count):
+        v = base[j]
+        v[0] = rng.choice([0.3, 0.45, 0.7, 0.9, 1.0, 1.6])
+        v[1] = rng.choice([2.0, 4.0, 6.0, 8.0, 12.0])
+        for s in range(S):
+            o = 2 + 6 * s
+            v[o + 2] = rng.choice([2.0, 8.0, 25.0, 45.0, 60.0])      # r
+            v[o + 5] = rng.choice([0.6, 1.0, 2.5, 5.0, 12.0])        # sigma
+            if j % 11 == 5 and s == S - 1:
+                v[o + 5] = 5e-5                                       # switched off
+    return base
+
+
+def test_sweep_against_gpu_fp64():
+    """48 vectors on two wedges, each evaluated by the resampled path and by
+    the GPU's FP64 path: every star within the bar and every integral within
+    2e-8.  A switched-off stream's integral is 0 in both; its stars are
+    compared only where the FP64 path's probability is finite."""
+    worst = dict(I_bg=0.0, I_st=0.0, lnP=0.0)
+    for k, shape in enumerate((LONG_R, synth.WedgeShape(350, 60, 24, 60, 1500))):
+        p, stars = synth.workload(150 + k, shape, 3)
+        vecs = sweep_vectors(p, 24, 7000 + k)
+        ph, sh = ParamsHolder(p), StarsHolder(stars)
+        with sep.Context(ph, sh) as ctx:
+            got = ctx.eval(vecs, precision=sep.PREC_RESAMPLED, per_star=True)
+            ref = ctx.eval(vecs, precision=sep.PREC_FP64, per_star=True)
+        for g, r in zip(got, ref):
+            worst["I_bg"] = max(worst["I_bg"], rel(g.background_integral, r.background_integral))
+            live = r.stream_integrals != 0.0
+            assert np.all(g.stream_integrals[~live] == 0.0)
+            if live.any():
+                worst["I_st"] = max(worst["I_st"], rel(g.stream_integrals[live], r.stream_integrals[live]))
+            ok = np.isfinite(r.log10_p)
+            assert np.array_equal(ok, np.isfinite(g.log10_p))
+            if ok.any():
+                worst["lnP"] = max(worst["lnP"], math.log(10) * float(np.max(np.abs(g.log10_p[ok] - r.log10_p[ok]))))
+    print("worst over 48 vectors:", {k: f"{v:.2e}" for k, v in worst.items()})
+    assert worst["lnP"] <= LNP_TOL
+    assert worst["I_bg"] <= INT_REL_TOL and worst["I_st"] <= INT_REL_TOL
+
+
 def test_full_size_against_gpu_fp64():
     p, stars = synth.workload(0, synth.FULL_SIZE, synth.FULL_SIZE_STREAMS)
     print("full size:", len(stars), "stars", flush=True)
