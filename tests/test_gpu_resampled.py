@@ -282,6 +282,32 @@ def test_sweep_against_gpu_fp64():
     assert worst["I_bg"] <= INT_REL_TOL and worst["I_st"] <= INT_REL_TOL
 
 
+def test_sweep_over_stripes():
+    """The same comparison on seven stripes, from low to high latitude and
+    through the Galactic centre's direction (the magnitude grid's case),
+    6 vectors each with the extremes above."""
+    worst = dict(I_bg=0.0, I_st=0.0, lnP=0.0)
+    for i, wedge in enumerate((10, 20, 37, 52, 61, 79, 86)):
+        p, stars = synth.workload(160 + i, synth.WedgeShape(350, 60, 24, 60, 1500), 2)
+        p.wedge = wedge
+        vecs = sweep_vectors(p, 6, 8000 + i)
+        ph, sh = ParamsHolder(p), StarsHolder(stars)
+        with sep.Context(ph, sh) as ctx:
+            got = ctx.eval(vecs, precision=sep.PREC_RESAMPLED, per_star=True)
+            ref = ctx.eval(vecs, precision=sep.PREC_FP64, per_star=True)
+        for g, r in zip(got, ref):
+            e_bg = rel(g.background_integral, r.background_integral)
+            live = r.stream_integrals != 0.0
+            e_st = rel(g.stream_integrals[live], r.stream_integrals[live]) if live.any() else 0.0
+            ok = np.isfinite(r.log10_p)
+            assert np.array_equal(ok, np.isfinite(g.log10_p))
+            lnp = math.log(10) * float(np.max(np.abs(g.log10_p[ok] - r.log10_p[ok]))) if ok.any() else 0.0
+            worst = dict(I_bg=max(worst["I_bg"], e_bg), I_st=max(worst["I_st"], e_st), lnP=max(worst["lnP"], lnp))
+        print(f"stripe {wedge}: worst so far", {k: f"{v:.2e}" for k, v in worst.items()}, flush=True)
+    assert worst["lnP"] <= LNP_TOL
+    assert worst["I_bg"] <= INT_REL_TOL and worst["I_st"] <= INT_REL_TOL
+
+
 def test_full_size_against_gpu_fp64():
     p, stars = synth.workload(0, synth.FULL_SIZE, synth.FULL_SIZE_STREAMS)
     print("full size:", len(stars), "stars", flush=True)
