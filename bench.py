@@ -279,8 +279,11 @@ def bench_batch(args):
     head_prec = sep.PREC_RESAMPLED if args.batch_precision == "resampled" else sep.PREC_MIXED
     other_prec = sep.PREC_MIXED if head_prec == sep.PREC_RESAMPLED else sep.PREC_RESAMPLED
     tune_t0 = time.perf_counter()
-    choice = autotune.tune(groups, vecs, head_prec, proxy_vectors=1, reps=1,
-                           blocks_per_cu=(2, 4), use_cache=False, log=log)
+    if args.variant is not None:
+        choice = {"variant": args.variant[0], "blocks_per_cu": args.variant[1], "table": []}
+    else:
+        choice = autotune.tune(groups, vecs, head_prec, proxy_vectors=1, reps=1,
+                               blocks_per_cu=(2, 4), use_cache=False, log=log)
     tune_s = time.perf_counter() - tune_t0
     log(f"autotuner ({args.batch_precision}): variant {choice['variant']} blocks/CU "
         f"{choice['blocks_per_cu']} ({tune_s:.1f} s)")
@@ -455,6 +458,10 @@ def main():
                     help="precision of the headline batch (the other one runs beside it)")
     ap.add_argument("--no-secondary", action="store_true",
                     help="skip the second batch precision (for profiling one path)")
+    ap.add_argument("--variant", type=int, nargs=2, metavar=("V", "BLOCKS_PER_CU"), default=None,
+                    help="pin the headline batch's kernel variant and blocks per CU instead of "
+                         "tuning (counter runs of one kernel: the tuner's timings move under "
+                         "a profiler)")
     ap.add_argument("--serving-requests", type=int, default=40)
     ap.add_argument("--serving-rates", type=float, nargs="+", default=[10.0, 50.0, 200.0, 800.0],
                     help="Poisson arrival rates (requests/s) of the serving sweep")
