@@ -112,6 +112,8 @@ int build_context(mw_ctx* c, const std::vector<RegionGeom>& geoms, int n_conv, i
         }
         tabulate_bg_width(*rd, t.col.data(), t.col.data() + t.n_cols, t.col.data() + 2 * t.n_cols,
                           g.mu_steps, g.nu_steps);
+        const char* bg_env = std::getenv("MW_HIP_BG_GRID");
+        rd->bg_grid = !(bg_env && std::strcmp(bg_env, "0") == 0);
         rd->h_rq = std::move(t.rq);
         rd->h_rstep = std::move(t.rstep);
         c->regions.push_back(std::move(rd));
@@ -173,9 +175,9 @@ int run_plan(mw_ctx* c, Plan& P)
             Group& g = igroups.back();
             RegionDev& rd = *c->regions[pr.region];
             TableRef tr;
-            const int level = select_level(rd, P.K[pr.slot], P.resample);
-            const int layout = level >= 0 ? table_layout(rd, level, P.K[pr.slot]) : 0;
-            if ((rc = region_tables(rd, level, layout, &tr))) return rc;
+            const GridPlan gp = plan_grid(rd, P.K[pr.slot], P.resample);
+            const int level = gp.level, layout = gp.layout;
+            if ((rc = region_tables(rd, gp, &tr))) return rc;
             RegionLaunch& L = c->h_ritems.p[item];
             L.rq = tr.rq;
             L.rstep = tr.rstep;
@@ -202,6 +204,8 @@ int run_plan(mw_ctx* c, Plan& P)
             L.grid_step = layout > 0 ? recurrence_step(level, P.K[pr.slot]) : 0.0;
             L.grid_lo = tr.lo;
             L.bg_stride = layout_bg_stride(layout);
+            L.bg_count = (int)tr.bg_count;
+            L.bg_rq = tr.bg_rq;
             if (L.grid_step > 0.0) points_rec += rd.n_cols * tr.nodes;
             points_bg += (level >= 0 ? rd.n_cols * tr.bg_nodes : (pr.end - pr.begin) * (long)rd.n_conv);
             ReduceItem& R = c->h_rred.p[item];
@@ -481,6 +485,8 @@ size_t mw_ctx_resident_bytes(const mw_ctx* c)
         for (const auto& per_level : r->levels)
             for (const auto& q : per_level)
                 if (q) b += q->rq.cap * sizeof(double) + q->rstep.cap * sizeof(RStep);
+        for (const auto& q : r->bg_levels)
+            if (q) b += q->rq.cap * sizeof(double) + q->rstep.cap * sizeof(RStep);
     }
     b += (c->star.cap + c->ratio.cap + c->kw.cap) * sizeof(double);
     return b;

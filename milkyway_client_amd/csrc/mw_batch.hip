@@ -35,12 +35,12 @@ int BatchRunner::init(int device, int flags)
     return MW_OK;
 }
 
-// Quadrature table of region rd under the model k, as one number: level and
-// layout.
+// Quadrature tables of region rd under the model k, as one number: level,
+// layout and background table.
 int BatchRunner::table_choice(const RegionDev& rd, const EvalConst& k) const
 {
-    const int level = select_level(rd, k, resample_);
-    return (level + 1) * kTableLayouts + (level >= 0 ? table_layout(rd, level, k) : 0);
+    const GridPlan g = plan_grid(rd, k, resample_);
+    return ((g.level + 1) * kTableLayouts + g.layout) * (kBgLevels + 1) + g.bg_level + 1;
 }
 
 bool BatchRunner::same_modes(const std::vector<Slot>& slots) const
@@ -111,9 +111,9 @@ int BatchRunner::plan(const std::vector<Slot>& slots, int precision)
         slot_stars_[s] = c->n_stars;
         for (const auto& rd : c->regions) {
             TableRef tr;
-            const int level = select_level(*rd, slots[s].k, resample_);
-            const int layout = level >= 0 ? table_layout(*rd, level, slots[s].k) : 0;
-            int rc0 = region_tables(*rd, level, layout, &tr);
+            const GridPlan gp = plan_grid(*rd, slots[s].k, resample_);
+            const int level = gp.level, layout = gp.layout;
+            int rc0 = region_tables(*rd, gp, &tr);
             if (rc0) return rc0;
             levels_.push_back(table_choice(*rd, slots[s].k));
             points_ += rd->n_cols * tr.nodes;
@@ -134,9 +134,11 @@ int BatchRunner::plan(const std::vector<Slot>& slots, int precision)
             L.grid_step = layout > 0 ? recurrence_step(level, slots[s].k) : 0.0;
             L.grid_lo = tr.lo;
             L.bg_stride = layout_bg_stride(layout);
+            L.bg_count = (int)tr.bg_count;
+            L.bg_rq = tr.bg_rq;
             if (L.grid_step > 0.0) points_rec_ += rd->n_cols * tr.nodes;
             points_bg_ += rd->n_cols * tr.bg_nodes;
-            items.push_back(It{ L, (double)L.r_steps * L.n_conv * (2.0 + L.n_streams) });
+            items.push_back(It{ L, (double)L.r_steps * L.n_conv * (2.0 + L.n_streams) + 6.0 * L.bg_count });
         }
     }
     n_ritems_ = (int)items.size();

@@ -252,9 +252,17 @@ constexpr int kQuadLevels = 13;
 constexpr int kQuadTables = 2 * kQuadLevels;
 // Layouts of a table: 0 = (r, weight) pairs for kernels that evaluate each
 // node's exponential; 1, 2, 3 = the recurrence layout on distance grids with
-// the background on every 1st, 2nd, 4th node.
-constexpr int kTableLayouts = 4;
-inline int layout_bg_stride(int layout) { return layout == 0 ? 0 : 1 << (layout - 1); }
+// the background on every 1st, 2nd, 4th node; 4 = the recurrence on a
+// distance grid for the streams alone, the background on a grid of its own
+// uniform in magnitude (a background table, index t < kBgLevels: spacing
+// 0.25 * 2^(-t/2) mag).
+constexpr int kTableLayouts = 5;
+constexpr int kLayoutSplit = 4;
+constexpr int kBgLevels = 13;
+inline int layout_bg_stride(int layout)
+{
+    return (layout == 0 || layout == kLayoutSplit) ? 0 : 1 << (layout - 1);
+}
 struct ResampledDev {
     int r_steps = 0, n_conv = 0;
     long nodes = 0;
@@ -282,7 +290,15 @@ struct RegionDev {
     // (kpc), tabulated at q = 2^(j / 4 - kWidthQ0), from a grid of sights
     // that includes the region's edges (see bg_width).
     std::vector<float> bg_width_q;
+    // Smallest angle (radians) at which the background's singularities sit
+    // off the real axis of the magnitude variable along the region's sights,
+    // at the same q (see bg_theta).
+    std::vector<float> bg_theta_q;
+    // Layout 4 allowed (off when the environment has MW_HIP_BG_GRID=0 as the
+    // region is built: the background then stays on the distance grid).
+    bool bg_grid = true;
     std::unique_ptr<ResampledDev> levels[kQuadTables][kTableLayouts];
+    std::unique_ptr<ResampledDev> bg_levels[kBgLevels];
     std::mutex level_mu;
 };
 
@@ -296,16 +312,30 @@ struct TableRef {
     int layout;        // see kTableLayouts
     double lo;         // resampled: the grid's first node
     long bg_nodes;     // background evaluations per sight (nodes / stride in the recurrence layouts)
+    const double* bg_rq = nullptr;   // layout 4: the background grid's (r, weight) entries
+    long bg_count = 0;               // and their number (rows of kResampleRow, padding weightless)
 };
 
-// Tabulate RegionDev::bg_width_q from the region's sight directions
-// (ux, uy, uz arrays of n_cols over mu_steps x nu_steps, mu-major).
+// What a region is evaluated on under one model: the quadrature table index
+// (-1 = the exact tables), its layout and, for layout 4, the background
+// table index.
+struct GridPlan {
+    int level = -1, layout = 0, bg_level = -1;
+};
+
+// Tabulate RegionDev::bg_width_q and bg_theta_q from the region's sight
+// directions (ux, uy, uz arrays of n_cols over mu_steps x nu_steps, mu-major).
 constexpr int kWidthQ = 33, kWidthQ0 = 4;
 void tabulate_bg_width(RegionDev& rd, const double* ux, const double* uy, const double* uz, int mu_steps,
                        int nu_steps);
 // A lower bound on the width of the background's peak along any sight of the
 // region for flattening q (kpc).
 double bg_width(const RegionDev& rd, double q);
+// A lower bound on that angle over the region's sights for flattening q.
+double bg_theta(const RegionDev& rd, double q);
+// Background table index for flattening q (the coarsest spacing within
+// 0.15 x 2.17 x theta and 0.25 mag), or -1 if none is fine enough.
+int bg_grid_level(const RegionDev& rd, double q);
 
 // Resampled table index to use for region rd under the model k, or -1 for
 // the exact tables (resampling off, a background mode it does not cover, or
@@ -322,8 +352,13 @@ double recurrence_step(int level, const EvalConst& k);
 // widest background stride (1, 2, 4 nodes) whose spacing still meets the
 // background's own bound (0.15 b and 2 kpc).
 int table_layout(const RegionDev& rd, int level, const EvalConst& k);
-// The tables of that level and layout, built and uploaded on first use.
-int region_tables(RegionDev& rd, int level, int layout, TableRef* out);
+// The grid a region runs on under the model k: select_level's table and
+// table_layout's layout, or layout 4 (streams on a distance grid spaced for
+// the streams alone, background on its own magnitude grid) where a count
+// of vector instructions per sight says it is cheaper.  Pure.
+GridPlan plan_grid(const RegionDev& rd, const EvalConst& k, bool resample);
+// The tables of that plan, built and uploaded on first use.
+int region_tables(RegionDev& rd, const GridPlan& g, TableRef* out);
 
 }  // namespace mw
 

@@ -86,7 +86,10 @@ struct RegionLaunch {
                                // recurrence layout (mixed kernels take the stream Gaussians by
                                // recurrence); node m of the region sits at grid_lo + m grid_step
     double grid_lo;
-    int bg_stride;             // recurrence layout: background on every bg_stride-th node
+    int bg_stride;             // recurrence layout: background on every bg_stride-th node, or
+                               // 0: on its own grid, the bg_count (r, weight) entries of bg_rq
+    int bg_count;
+    const double* bg_rq;
 };
 
 // Stars resident on the device, SoA.

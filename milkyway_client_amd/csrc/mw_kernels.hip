@@ -327,7 +327,7 @@ __device__ __forceinline__ void row_by_recurrence(const double2* row, int n, dou
             const double2 e = row[i];
             bg_point<BG>(e.x, e.y, v.bg, K, acc[0]);
         }
-    } else {
+    } else if constexpr (Q > 1) {
         // (stream weight, background weight) entries; node i sits at r0 + i h
         // and only every Q-th node carries a background weight
         double rb = r0;
@@ -336,6 +336,8 @@ __device__ __forceinline__ void row_by_recurrence(const double2* row, int n, dou
             rb += Q * h;
         }
     }
+    // Q == 0: (r, weight) entries for the streams; the background runs on a
+    // grid of its own.
 #pragma unroll
     for (int s = 0; s < S; s++) {
         const double p = v.st[s].p;
@@ -356,7 +358,7 @@ __device__ __forceinline__ void row_by_recurrence(const double2* row, int n, dou
         const double c = C[s];
 #pragma unroll 4
         for (int i = 0; i < n; i++) {
-            a = fma(Q == 1 ? row[i].y : row[i].x, g, a);
+            a = fma(Q <= 1 ? row[i].y : row[i].x, g, a);
             g *= rr;
             rr *= c;
         }
@@ -368,7 +370,9 @@ __device__ __forceinline__ void row_by_recurrence(const double2* row, int n, dou
 
 // One logical block of a resampled region on a distance grid (grid_step > 0;
 // the region is evaluated whole, from tables in the recurrence layout): the
-// streams by recurrence, the background on every Q-th node.  The resampled
+// streams by recurrence, the background on every Q-th node, or (Q == 0) on
+// its own grid uniform in magnitude, whose bg_count (r, weight) entries pass
+// through the LDS tile first.  The resampled
 // tables carry every weight in the node entries (each row's irv and reff
 // are 1), so the column sums take the terms directly, plainly summed (a
 // column has a few hundred nodes, not the exact quadrature's tens of
@@ -405,6 +409,20 @@ __device__ __forceinline__ void integral_block_recurrence(const RegionLaunch& it
             const double lo = min_t2(fma(sight.st[s].p, it.grid_lo, -sight.st[s].m),
                                      fma(sight.st[s].p, r_last, -sight.st[s].m));
             floor_t2[s] = (float)(lo + kNegligibleLog2);   // rounding either way moves the bound by < 1e-5
+        }
+        if constexpr (Q == 0) {
+            const double2* bq = reinterpret_cast<const double2*>(it.bg_rq);
+            for (int e0 = 0; e0 < it.bg_count; e0 += kTileD2) {
+                const int m = (it.bg_count - e0 < kTileD2) ? it.bg_count - e0 : kTileD2;
+                __syncthreads();
+                for (int e = threadIdx.x; e < m; e += kBlock) tile[e] = bq[e0 + e];
+                __syncthreads();
+#pragma unroll 2
+                for (int i = 0; i < m; i++) {
+                    const double2 e = tile[i];
+                    bg_point<BG>(e.x, e.y, sight.bg, K, acc[0]);
+                }
+            }
         }
         const int rt = kTileD2 / n;
         const double2* rq = reinterpret_cast<const double2*>(it.rq);
@@ -458,7 +476,8 @@ __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
 {
     if constexpr (MIXED) {
         if (it.grid_step > 0.0) {
-            if (it.bg_stride == 4) integral_block_recurrence<S, BG, 4>(it, lb, consts, prow, tile, red);
+            if (it.bg_stride == 0) integral_block_recurrence<S, BG, 0>(it, lb, consts, prow, tile, red);
+            else if (it.bg_stride == 4) integral_block_recurrence<S, BG, 4>(it, lb, consts, prow, tile, red);
             else if (it.bg_stride == 2) integral_block_recurrence<S, BG, 2>(it, lb, consts, prow, tile, red);
             else integral_block_recurrence<S, BG, 1>(it, lb, consts, prow, tile, red);
             return;

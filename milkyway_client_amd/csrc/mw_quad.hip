@@ -36,6 +36,7 @@ namespace {
 constexpr double kMagPerLn = 2.1714724095162588;   // 5 / ln 10
 constexpr double kDeltaMag = 0.02;                  // coarsest magnitude spacing
 constexpr double kDeltaKpc = 2.0;                   // coarsest distance spacing
+constexpr double kDeltaBgMag = 0.25;                // coarsest background-grid spacing
 
 // Spacing of table index t (see kQuadLevels).
 double table_delta(int t)
@@ -73,6 +74,19 @@ static double sight_bg_width(double ux, double uy, double uz, double q)
     return ux > 0.0 ? kSunR0 * std::sqrt(side) / P : kSunR0 / std::sqrt(P);
 }
 
+// In the magnitude variable x = 2.17 ln r the same singularities,
+// r = r_c +- i rg_min / sqrt(P), sit at Im x = +-2.17 theta with
+// sin theta = rg_min / 8.5 (P r_c^2 + rg_min^2 = 8.5^2): theta is the
+// angle, seen from the Sun, between the sight and the nearest point of the
+// profile's centre in the q-scaled metric.  A sight pointing away from the
+// centre has theta >= pi / 2.
+static double sight_bg_theta(double ux, double uy, double uz, double q)
+{
+    if (!(ux > 0.0)) return 0.5 * M_PI;
+    const double side = uy * uy + uz * uz / (q * q);
+    return std::asin(std::min(1.0, std::sqrt(side / (ux * ux + side))));
+}
+
 void tabulate_bg_width(RegionDev& rd, const double* ux, const double* uy, const double* uz, int mu_steps,
                        int nu_steps)
 {
@@ -84,15 +98,18 @@ void tabulate_bg_width(RegionDev& rd, const double* ux, const double* uy, const 
     for (int i = 0; i < nm; i++) rows.push_back(nm > 1 ? (long)i * (mu_steps - 1) / (nm - 1) : 0);
     for (int j = 0; j < nn; j++) cols.push_back(nn > 1 ? (long)j * (nu_steps - 1) / (nn - 1) : 0);
     rd.bg_width_q.assign(kWidthQ, INFINITY);
+    rd.bg_theta_q.assign(kWidthQ, INFINITY);
     for (int t = 0; t < kWidthQ; t++) {
         const double q = std::exp2(t / 4.0 - kWidthQ0);
-        double w = INFINITY;
+        double w = INFINITY, th = INFINITY;
         for (long r : rows)
             for (long c : cols) {
                 const long i = r * nu_steps + c;
                 w = std::min(w, sight_bg_width(ux[i], uy[i], uz[i], q));
+                th = std::min(th, sight_bg_theta(ux[i], uy[i], uz[i], q));
             }
         rd.bg_width_q[t] = (float)w;
+        rd.bg_theta_q[t] = (float)th;
     }
 }
 
@@ -109,6 +126,34 @@ double bg_width(const RegionDev& rd, double q)
     const int t = std::min((int)x, kWidthQ - 2);
     const double w = 0.85 * std::min(rd.bg_width_q[t], rd.bg_width_q[t + 1]);
     return std::max(w, bound);
+}
+
+double bg_theta(const RegionDev& rd, double q)
+{
+    // As bg_width: 0.85 of the smaller tabulated value around q, and never
+    // below the closed form asin(min sin(GC angle) min(1, 1/q)) (side / P
+    // grows with side, and side >= (1 - u_x^2) min(1, 1/q^2)).
+    const double bound = std::asin(std::min(1.0, rd.min_sin_gc * std::min(1.0, 1.0 / q)));
+    if (rd.bg_theta_q.size() != (size_t)kWidthQ || !(q > 0.0)) return bound;
+    const double x = (std::log2(q) + kWidthQ0) * 4.0;
+    if (!(x >= 0.0) || !(x <= kWidthQ - 1)) return bound;
+    const int t = std::min((int)x, kWidthQ - 2);
+    return std::max(0.85 * std::min(rd.bg_theta_q[t], rd.bg_theta_q[t + 1]), bound);
+}
+
+int bg_grid_level(const RegionDev& rd, double q)
+{
+    // 16-node stencils on a grid uniform in x = 2.17 ln r: spacing 0.15 of
+    // the singularities' distance from the real axis (2.17 theta), and at
+    // most 0.25 mag, where the profile's r^-4 tail (an exponential of rate
+    // 1.84 per mag in x) sets the interpolation error instead.  Measured on
+    // the fold of a dense quadrature over 1-105 kpc, every (rg_min, P, r0)
+    // from (0.5-8.4 kpc, 0.5-5, 1-14 kpc), towards and away from the
+    // centre: relative error of the integral below 5.2e-12.
+    const double dg = std::min(0.15 * kMagPerLn * bg_theta(rd, q), kDeltaBgMag);
+    if (!(dg > 0.0)) return -1;
+    const int t = level_for(dg, kDeltaBgMag);
+    return t < kBgLevels ? t : -1;
 }
 
 int select_level(const RegionDev& rd, const EvalConst& k, bool resample)
@@ -169,39 +214,132 @@ int table_layout(const RegionDev& rd, int level, const EvalConst& k)
     return 1;
 }
 
-int region_tables(RegionDev& rd, int level, int layout, TableRef* out)
+namespace {
+
+// Vector-instruction cycles per sight and grid node of the integral kernels
+// (from their AMD ISA and the VALU counters): a node of the recurrence
+// layouts, each stream's recurrence step, a background evaluation (two
+// quarter-rate FP64 transcendentals among its 15.5 instructions) and each
+// stream's FP32 exponential where there is no recurrence.
+constexpr double kCostNode = 1.5, kCostRecStream = 3.25, kCostBg = 21.5, kCostExpStream = 7.0;
+
+long padded_rows(long n) { return (n + kResampleRow - 1) / kResampleRow * kResampleRow; }
+
+// Nodes of the resampled table `level` over the region (as the builder
+// makes them: the fold's lattice, padded to whole rows).
+long table_nodes(const RegionDev& rd, int level, int stride)
 {
+    const int half = kResampleDegree / 2;
+    if (level < kQuadLevels) {
+        const double d = table_delta(level);
+        return padded_rows((long)std::ceil(rd.g_span / d) + 2 * half + 3);
+    }
+    const double d = table_delta(level);
+    const double span = rd.r_span + (half + 1) * stride * d;
+    const long M = std::max((long)std::ceil(span / d) + half + 2,
+                            ((long)std::ceil(span / (stride * d)) + half + 1) * stride + 1);
+    return padded_rows(M);
+}
+
+long bg_table_nodes(const RegionDev& rd, int bg_level)
+{
+    const int half = kResampleDegree / 2;
+    const double d = kDeltaBgMag * std::pow(2.0, -0.5 * bg_level);
+    return padded_rows((long)std::ceil(rd.g_span / d) + 2 * half + 3);
+}
+
+}  // namespace
+
+GridPlan plan_grid(const RegionDev& rd, const EvalConst& k, bool resample)
+{
+    GridPlan g;
+    g.level = select_level(rd, k, resample);
+    if (g.level < 0) return g;
+    g.layout = table_layout(rd, g.level, k);
+    if (k.bg_mode != kBgHernquist || !rd.bg_grid) return g;
+    const int S = k.n_streams;
+    double cost;
+    if (g.layout > 0) {
+        const int q = layout_bg_stride(g.layout);
+        const long n = table_nodes(rd, g.level, q);
+        cost = n * (kCostNode + kCostRecStream * S) + (double)((n + q - 1) / q) * kCostBg;
+    } else {
+        cost = table_nodes(rd, g.level, 1) * (kCostBg + kCostExpStream * S);
+    }
+    // The split: a distance grid spaced for the streams alone, and the
+    // background grid.
+    double sigma = INFINITY;
+    for (int s = 0; s < k.n_streams; s++) {
+        if (k.on[s] == 0.0 || !(k.inv_2s2[s] > 0.0)) continue;
+        sigma = std::min(sigma, std::sqrt(0.5 / k.inv_2s2[s]));
+    }
+    const int Ls = level_for(0.3 * sigma, kDeltaKpc);
+    const int bl = bg_grid_level(rd, k.q);
+    if (Ls >= kQuadLevels || bl < 0) return g;
+    const int level = kQuadLevels + Ls;
+    if (!(recurrence_step(level, k) > 0.0)) return g;
+    const double split = table_nodes(rd, level, 1) * (kCostNode + kCostRecStream * S) +
+                         bg_table_nodes(rd, bl) * kCostBg;
+    if (split < 0.9 * cost) g = GridPlan{ level, kLayoutSplit, bl };
+    return g;
+}
+
+static int upload_table(const RegionTables& t, long M, std::unique_ptr<ResampledDev>* slot)
+{
+    auto q = std::make_unique<ResampledDev>();
+    q->r_steps = t.r_steps;
+    q->n_conv = t.n_conv;
+    q->nodes = M;
+    q->lo = t.lo;
+    int rc;
+    if ((rc = q->rq.reserve(t.rq.size())) || (rc = q->rstep.reserve(t.rstep.size()))) return rc;
+    MW_HIP_CHECK(hipMemcpy(q->rq.p, t.rq.data(), sizeof(double) * t.rq.size(), hipMemcpyHostToDevice));
+    MW_HIP_CHECK(hipMemcpy(q->rstep.p, t.rstep.data(), sizeof(RStep) * t.rstep.size(), hipMemcpyHostToDevice));
+    *slot = std::move(q);
+    return MW_OK;
+}
+
+int region_tables(RegionDev& rd, const GridPlan& g, TableRef* out)
+{
+    const int level = g.level, layout = g.layout;
     if (level < 0) {
         const long pts = (long)rd.geom.r_steps * rd.n_conv;
         *out = TableRef{ rd.rq.p, rd.rstep.p, rd.geom.r_steps, rd.n_conv, -1, pts, 0, 0.0, pts };
         return MW_OK;
     }
-    if (level >= kQuadTables || layout < 0 || layout >= kTableLayouts || (layout > 0 && level < kQuadLevels))
+    if (level >= kQuadTables || layout < 0 || layout >= kTableLayouts || (layout > 0 && level < kQuadLevels) ||
+        (layout == kLayoutSplit) != (g.bg_level >= 0) || g.bg_level >= kBgLevels)
         return fail(MW_E_INVALID, "no such quadrature table");
     std::lock_guard<std::mutex> lock(rd.level_mu);
-    // Layout 1 (stride 1) is the (r, weight) table of layout 0.
-    std::unique_ptr<ResampledDev>& slot = rd.levels[level][layout == 1 ? 0 : layout];
+    // Layouts 1 (stride 1) and 4 (the streams' table) are the (r, weight)
+    // table of layout 0.
+    const int stride = layout_bg_stride(layout);
+    std::unique_ptr<ResampledDev>& slot = rd.levels[level][stride > 1 ? layout : 0];
     if (!slot) {
         RegionTables t;
         const long M = build_resampled_tables(rd.h_rq, rd.h_rstep, rd.n_conv, table_delta(level),
-                                              level >= kQuadLevels, layout == 1 ? 0 : layout_bg_stride(layout),
-                                              &t);
+                                              level >= kQuadLevels, stride > 1 ? stride : 0, &t);
         if (M < 1) return fail(MW_E_INVALID, "resampled quadrature failed");
-        auto q = std::make_unique<ResampledDev>();
-        q->r_steps = t.r_steps;
-        q->n_conv = t.n_conv;
-        q->nodes = M;
-        q->lo = t.lo;
         int rc;
-        if ((rc = q->rq.reserve(t.rq.size())) || (rc = q->rstep.reserve(t.rstep.size()))) return rc;
-        MW_HIP_CHECK(hipMemcpy(q->rq.p, t.rq.data(), sizeof(double) * t.rq.size(), hipMemcpyHostToDevice));
-        MW_HIP_CHECK(hipMemcpy(q->rstep.p, t.rstep.data(), sizeof(RStep) * t.rstep.size(), hipMemcpyHostToDevice));
-        slot = std::move(q);
+        if ((rc = upload_table(t, M, &slot))) return rc;
     }
     const ResampledDev& q = *slot;
-    const int stride = layout_bg_stride(layout);
     *out = TableRef{ q.rq.p, q.rstep.p, q.r_steps, q.n_conv, level, q.nodes, layout, q.lo,
                      stride > 1 ? (q.nodes + stride - 1) / stride : q.nodes };
+    if (layout == kLayoutSplit) {
+        std::unique_ptr<ResampledDev>& bslot = rd.bg_levels[g.bg_level];
+        if (!bslot) {
+            RegionTables t;
+            const long M = build_resampled_tables(rd.h_rq, rd.h_rstep, rd.n_conv,
+                                                  kDeltaBgMag * std::pow(2.0, -0.5 * g.bg_level), false, 0, &t);
+            if (M < 1) return fail(MW_E_INVALID, "resampled quadrature failed");
+            int rc;
+            if ((rc = upload_table(t, M, &bslot))) return rc;
+        }
+        out->bg_rq = bslot->rq.p;
+        out->bg_count = (long)bslot->r_steps * bslot->n_conv;
+        out->bg_nodes = out->bg_count;
+    }
     return MW_OK;
 }
 

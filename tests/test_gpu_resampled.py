@@ -135,10 +135,33 @@ def background_stride_probe(ctx, vec):
     return tm.integral_points, tm.integral_points_background
 
 
-def test_background_stride():
-    """Where the background is wider than the streams it is sampled on every
-    2nd or 4th node of the distance grid: fewer background evaluations than
-    nodes, and the background integral still within the bar."""
+def test_background_grid():
+    """Where the background is wider than the streams it runs on a grid of
+    its own, uniform in magnitude: a fraction of the stream grid's nodes,
+    and the background integral within the fold's own error (1e-11, far
+    inside the bar) of the oracle."""
+    p, stars = synth.workload(91 + 3, LONG_R, 3)
+    for st in p.streams:      # narrow streams: a fine distance grid
+        st.parameters[4] = 1.0
+    ph, sh = ParamsHolder(p), StarsHolder(stars)
+    with sep.Context(ph, sh, timing=True) as ctx:
+        pts, bgp = background_stride_probe(ctx, p.parameter_vector())
+        got = ctx.eval(p.parameter_vector(), precision=sep.PREC_RESAMPLED, per_star=True)[0]
+    ref = O.evaluate_full(ParamsHolder(p), StarsHolder(stars))
+    e_bg = rel(got.background_integral, ref.background_integral)
+    lnp = math.log(10) * np.max(np.abs(got.log10_p - ref.log10_p))
+    print(f"{bgp} background evaluations for {pts} nodes; I_bg {e_bg:.2e}, lnP {lnp:.2e}")
+    assert bgp <= 0.25 * pts
+    assert e_bg <= 1e-11
+    assert lnp <= LNP_TOL
+
+
+def test_background_stride(monkeypatch):
+    """With the background's own grid switched off (MW_HIP_BG_GRID=0) it is
+    sampled on every 2nd or 4th node of the distance grid where it is wider
+    than the streams: fewer background evaluations than nodes, and the
+    background integral still within the bar."""
+    monkeypatch.setenv("MW_HIP_BG_GRID", "0")
     p, stars = synth.workload(91 + 3, LONG_R, 3)
     for st in p.streams:      # narrow streams: a fine grid, the background free to stride
         st.parameters[4] = 1.0
@@ -149,14 +172,17 @@ def test_background_stride():
     ref = O.evaluate_full(ParamsHolder(p), StarsHolder(stars))
     e_bg = rel(got.background_integral, ref.background_integral)
     print(f"{bgp} background evaluations for {pts} nodes; I_bg {e_bg:.2e}")
-    assert bgp <= 0.55 * pts
+    assert 0.2 * pts <= bgp <= 0.55 * pts
     assert e_bg <= INT_REL_TOL
 
 
-def test_graph_batch_replans_when_the_layout_changes():
-    """A graph-replayed batch given vectors whose tables differ only in the
-    background stride (same grid level, different q) must re-plan: each
-    run equals the single evaluations of its own vectors."""
+@pytest.mark.parametrize("bg_grid", ["1", "0"])
+def test_graph_batch_replans_when_the_layout_changes(bg_grid, monkeypatch):
+    """A graph-replayed batch given vectors whose tables differ only in how
+    the background is sampled (same stream grid, different q: a different
+    background grid, or with that grid off a different stride) must
+    re-plan: each run equals the single evaluations of its own vectors."""
+    monkeypatch.setenv("MW_HIP_BG_GRID", bg_grid)
     p, stars = synth.workload(131, LONG_R, 2)
     for st in p.streams:      # narrow streams: a fine grid, the background free to stride
         st.parameters[4] = 1.0
@@ -173,8 +199,9 @@ def test_graph_batch_replans_when_the_layout_changes():
         def stride(q):
             return round(seen[q][1][0] / seen[q][1][1])
         pairs = [(a, b) for a in seen for b in seen
-                 if abs(seen[a][1][0] - seen[b][1][0]) <= 0.1 * seen[a][1][0] and stride(a) != stride(b)]
-        assert pairs, f"no two q values share a level with different strides: {seen}"
+                 if abs(seen[a][1][0] - seen[b][1][0]) <= 0.1 * seen[a][1][0] and
+                 (stride(a) != stride(b) if bg_grid == "0" else seen[a][1][1] != seen[b][1][1])]
+        assert pairs, f"no two q values share a level with different background sampling: {seen}"
         qa, qb = pairs[0]
         va, vb = seen[qa][0], seen[qb][0]
         print(f"q {qa} vs {qb}: {seen[qa][1]} / {seen[qb][1]} (nodes, background evaluations)")
