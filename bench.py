@@ -97,18 +97,22 @@ def cpu_seconds_per_eval(p, stars, rows_per_region, n_star_sample):
 # Workloads
 # --------------------------------------------------------------------------
 
-def mixed_roofline(points: float, rec: float, n_streams: int, ms: float, bg: float = None) -> dict:
+def mixed_roofline(points: float, rec: float, n_streams: int, ms: float, bg: float = None,
+                   terms: float = None) -> dict:
     """Roofline of a mixed-precision integral launch, each evaluation priced
     by the method that ran it: the frozen per-point model (FP32 exp, against
-    the FP32 vector peak) for exponentials per point, the same weights over
-    the recurrence's operations (FP64, against the FP64 peak) on distance
-    grids, where the background is charged per background evaluation (bg,
-    every 1st, 2nd or 4th node).  frac is the time the launch would take at
-    those peaks over its measured time."""
+    the FP32 vector peak) for exponentials per point; on distance grids the
+    same weights over the recurrence's operations (FP64, against the FP64
+    peak): 4 per stream term (terms: the library's count of them, n_streams
+    per node on a shared grid, or the sum over streams with grids of their
+    own) and 35 per background evaluation (bg: every 1st, 2nd or 4th node,
+    or the background's own grid).  frac is the time the launch would take
+    at those peaks over its measured time."""
     bg_rec = rec if bg is None else bg - (points - rec)
+    if terms is None:
+        terms = rec * n_streams
     w_exp = (points - rec) * workmodel.flops_per_point(n_streams)
-    w_rec = (rec * workmodel.flops_per_point_recurrence(n_streams)
-             - (rec - bg_rec) * workmodel.flops_per_point(0))
+    w_rec = terms * workmodel.STREAM_RECURRENCE_OPS + bg_rec * workmodel.flops_per_point(0)
     t_peak = (w_exp / FP32_PEAK_TFLOPS + w_rec / FP64_PEAK_TFLOPS) / 1e12
     work = w_exp + w_rec
     return {"achieved": round(work / (ms * 1e-3) / 1e12, 3),
@@ -116,14 +120,14 @@ def mixed_roofline(points: float, rec: float, n_streams: int, ms: float, bg: flo
             "unit": "TFLOP/s", "frac": round(t_peak / (ms * 1e-3), 4),
             "integrand_evaluations_per_launch": int(points),
             "of_which_by_recurrence": int(rec),
+            "stream_terms_by_recurrence": int(terms),
             "background_evaluations": int(bg_rec + points - rec),
             "algorithmic_flop_eq_per_launch": work,
             "pricing": (f"{workmodel.flops_per_point(n_streams)} flop-eq per evaluation with an exp "
-                        f"per stream at {FP32_PEAK_TFLOPS} TFLOP/s (FP32), "
-                        f"{workmodel.flops_per_point_recurrence(n_streams)} by recurrence at "
-                        f"{FP64_PEAK_TFLOPS} (FP64), of which the background's "
-                        f"{workmodel.flops_per_point(0)} per background evaluation; peak is their "
-                        f"work-weighted mix")}
+                        f"per stream at {FP32_PEAK_TFLOPS} TFLOP/s (FP32); by recurrence "
+                        f"{workmodel.STREAM_RECURRENCE_OPS} per stream term and "
+                        f"{workmodel.flops_per_point(0)} per background evaluation at "
+                        f"{FP64_PEAK_TFLOPS} (FP64); peak is their work-weighted mix")}
 
 
 def build_batch():
@@ -291,7 +295,7 @@ def bench_batch(args):
     value, lat, elapsed, tm = timed_batch(pairs, vecs, head_prec, choice, args.steps, args.warmup)
     int_ms, points = tm.integral_ms, tm.integral_points
     roof = mixed_roofline(float(points), float(tm.integral_points_recurrence), BATCH_STREAMS, int_ms,
-                          float(tm.integral_points_background))
+                          float(tm.integral_points_background), float(tm.integral_stream_terms))
     log(f"  eager: persistent integral launch {int_ms:.2f} ms of {tm.total_ms:.2f} ms, "
         f"{points:.3e} integrand evaluations")
 
@@ -361,7 +365,8 @@ def bench_batch(args):
         v2, lat2, tm2, tune2 = second
         q2 = "exact" if quad == "resampled" else "resampled"
         roof2 = mixed_roofline(float(tm2.integral_points), float(tm2.integral_points_recurrence),
-                               BATCH_STREAMS, tm2.integral_ms, float(tm2.integral_points_background))
+                               BATCH_STREAMS, tm2.integral_ms, float(tm2.integral_points_background),
+                               float(tm2.integral_stream_terms))
         key2 = "mixed" if q2 == "exact" else "resampled"
         out[f"batched_mixed_{q2}_quadrature"] = {
             "value": round(v2, 4), "unit": "evals/s", "ratio_over_plain_fp64": round(v2 / plain, 3),

@@ -130,8 +130,12 @@ typedef struct {
                                           stream terms came from the FP64 recurrence instead of
                                           an exponential per point (resampled only) */
     long integral_points_background;   /* background evaluations: integral_points, except on
-                                          distance grids whose background is wide enough to be
-                                          sampled on every 2nd or 4th node */
+                                          distance grids whose background is sampled on every
+                                          2nd or 4th node or on a grid of its own */
+    long integral_stream_terms;        /* stream terms the recurrence ran over: per sight, the
+                                          grid's nodes times the streams, or, where each stream
+                                          has a grid of its own, the sum of those grids' nodes
+                                          (rows a stream does not reach still counted) */
 } mw_timing;
 
 /* Flags for mw_ctx_create */

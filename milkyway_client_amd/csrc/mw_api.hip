@@ -155,7 +155,7 @@ int run_plan(mw_ctx* c, Plan& P)
     struct Group { int bg; int first_item, n_items, first_block, n_blocks; };
     std::vector<Group> igroups;
     int total_iblocks = 0;
-    long points = 0, points_rec = 0, points_bg = 0;
+    long points = 0, points_rec = 0, points_bg = 0, stream_terms = 0;
     if (P.integrals && n_pr > 0) {
         if ((rc = c->ritems.reserve(n_pr)) || (rc = c->h_ritems.reserve(n_pr)) ||
             (rc = c->rred.reserve(n_pr)) || (rc = c->h_rred.reserve(n_pr)) ||
@@ -206,7 +206,11 @@ int run_plan(mw_ctx* c, Plan& P)
             L.bg_stride = layout_bg_stride(layout);
             L.bg_count = (int)tr.bg_count;
             L.bg_rq = tr.bg_rq;
-            if (L.grid_step > 0.0) points_rec += rd.n_cols * tr.nodes;
+            copy_stream_tables(tr, &L);
+            if (L.grid_step > 0.0) {
+                points_rec += rd.n_cols * tr.nodes;
+                stream_terms += rd.n_cols * (layout == kLayoutSplit ? tr.stream_terms : tr.nodes * c->S);
+            }
             points_bg += (level >= 0 ? rd.n_cols * tr.bg_nodes : (pr.end - pr.begin) * (long)rd.n_conv);
             ReduceItem& R = c->h_rred.p[item];
             R.block_begin = total_iblocks;       // global partial index
@@ -346,6 +350,7 @@ int run_plan(mw_ctx* c, Plan& P)
     c->timing.integral_points = points;
     c->timing.integral_points_recurrence = points_rec;
     c->timing.integral_points_background = points_bg;
+    c->timing.integral_stream_terms = stream_terms;
     c->timing.star_blocks = total_sblocks;
     return MW_OK;
 }

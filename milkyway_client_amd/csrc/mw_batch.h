@@ -62,6 +62,7 @@ public:
     long integral_points() const { return points_; }
     long integral_points_recurrence() const { return points_rec_; }
     long integral_points_background() const { return points_bg_; }
+    long integral_stream_terms() const { return terms_; }
     long star_blocks() const { return total_sblocks_; }
 
     static size_t scratch_bytes_for(const std::vector<Slot>& slots);
@@ -80,6 +81,7 @@ private:
     long points_ = 0;                // integrand evaluations of the planned integral pass
     long points_rec_ = 0;            // of those, on distance grids by recurrence
     long points_bg_ = 0;             // background evaluations (strided on distance grids)
+    long terms_ = 0;                 // stream terms by recurrence
     std::vector<int> slot_bg_, slot_n_regions_, slot_first_row_, slot_S_;
     std::vector<Group> igroups_, sgroups_;
     int n_ritems_ = 0, total_iblocks_ = 0, total_sblocks_ = 0;

@@ -40,7 +40,9 @@ int BatchRunner::init(int device, int flags)
 int BatchRunner::table_choice(const RegionDev& rd, const EvalConst& k) const
 {
     const GridPlan g = plan_grid(rd, k, resample_);
-    return ((g.level + 1) * kTableLayouts + g.layout) * (kBgLevels + 1) + g.bg_level + 1;
+    int key = ((g.level + 1) * kTableLayouts + g.layout) * (kBgLevels + 1) + g.bg_level + 1;
+    for (int s = 0; s < kMaxStreams; s++) key = key * (kQuadLevels + 1) + g.st_level[s] + 1;
+    return key;
 }
 
 bool BatchRunner::same_modes(const std::vector<Slot>& slots) const
@@ -88,6 +90,7 @@ int BatchRunner::plan(const std::vector<Slot>& slots, int precision)
     points_ = 0;
     points_rec_ = 0;
     points_bg_ = 0;
+    terms_ = 0;
     slot_bg_.assign(n, 0);
     slot_n_regions_.assign(n, 0);
     slot_first_row_.assign(n, 0);
@@ -136,9 +139,18 @@ int BatchRunner::plan(const std::vector<Slot>& slots, int precision)
             L.bg_stride = layout_bg_stride(layout);
             L.bg_count = (int)tr.bg_count;
             L.bg_rq = tr.bg_rq;
-            if (L.grid_step > 0.0) points_rec_ += rd->n_cols * tr.nodes;
+            copy_stream_tables(tr, &L);
+            const long terms = layout == kLayoutSplit ? tr.stream_terms : tr.nodes * L.n_streams;
+            if (L.grid_step > 0.0) {
+                points_rec_ += rd->n_cols * tr.nodes;
+                terms_ += rd->n_cols * terms;
+            }
             points_bg_ += rd->n_cols * tr.bg_nodes;
-            items.push_back(It{ L, (double)L.r_steps * L.n_conv * (2.0 + L.n_streams) + 6.0 * L.bg_count });
+            // cost per block: stream terms and background evaluations (about
+            // 6.5 stream terms each), or per point on the other layouts
+            const double cost = layout == kLayoutSplit ? terms + 6.5 * tr.bg_count
+                                                       : (double)L.r_steps * L.n_conv * (2.0 + L.n_streams);
+            items.push_back(It{ L, cost });
         }
     }
     n_ritems_ = (int)items.size();
@@ -492,6 +504,7 @@ int mw_batch_run(mw_batch* b, const double* params, mw_result* results)
     b->timing.integral_points = b->runner.integral_points();
     b->timing.integral_points_recurrence = b->runner.integral_points_recurrence();
     b->timing.integral_points_background = b->runner.integral_points_background();
+    b->timing.integral_stream_terms = b->runner.integral_stream_terms();
     b->timing.star_blocks = b->runner.star_blocks();
     return MW_OK;
 }

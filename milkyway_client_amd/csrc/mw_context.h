@@ -314,6 +314,11 @@ struct TableRef {
     long bg_nodes;     // background evaluations per sight (nodes / stride in the recurrence layouts)
     const double* bg_rq = nullptr;   // layout 4: the background grid's (r, weight) entries
     long bg_count = 0;               // and their number (rows of kResampleRow, padding weightless)
+    long stream_terms = 0;           // layout 4: stream terms per sight (the streams' grids' nodes)
+    // layout 4: each stream's own distance table (st_rows 0: switched off)
+    const double* st_rq[kMaxStreams] = {};
+    double st_step[kMaxStreams] = {}, st_lo[kMaxStreams] = {};
+    int st_rows[kMaxStreams] = {};
 };
 
 // What a region is evaluated on under one model: the quadrature table index
@@ -321,6 +326,7 @@ struct TableRef {
 // table index.
 struct GridPlan {
     int level = -1, layout = 0, bg_level = -1;
+    int st_level[kMaxStreams] = { -1, -1, -1, -1 };   // layout 4: each stream's distance level
 };
 
 // Tabulate RegionDev::bg_width_q and bg_theta_q from the region's sight
@@ -353,12 +359,22 @@ double recurrence_step(int level, const EvalConst& k);
 // background's own bound (0.15 b and 2 kpc).
 int table_layout(const RegionDev& rd, int level, const EvalConst& k);
 // The grid a region runs on under the model k: select_level's table and
-// table_layout's layout, or layout 4 (streams on a distance grid spaced for
-// the streams alone, background on its own magnitude grid) where a count
-// of vector instructions per sight says it is cheaper.  Pure.
+// table_layout's layout, or layout 4 (each stream on a distance grid spaced
+// for that stream alone, the background on its own magnitude grid) where a
+// count of vector instructions per sight says it is cheaper.  Pure.
 GridPlan plan_grid(const RegionDev& rd, const EvalConst& k, bool resample);
 // The tables of that plan, built and uploaded on first use.
 int region_tables(RegionDev& rd, const GridPlan& g, TableRef* out);
+// Layout 4's per-stream tables into a launch item.
+inline void copy_stream_tables(const TableRef& tr, RegionLaunch* L)
+{
+    for (int s = 0; s < kMaxStreams; s++) {
+        L->st_rq[s] = tr.st_rq[s];
+        L->st_step[s] = tr.st_step[s];
+        L->st_lo[s] = tr.st_lo[s];
+        L->st_rows[s] = tr.st_rows[s];
+    }
+}
 
 }  // namespace mw
 

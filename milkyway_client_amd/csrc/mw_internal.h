@@ -90,6 +90,12 @@ struct RegionLaunch {
                                // 0: on its own grid, the bg_count (r, weight) entries of bg_rq
     int bg_count;
     const double* bg_rq;
+    // bg_stride 0: each stream on a distance grid of its own, st_rows rows of
+    // kResampleRow (r, weight) entries from st_rq[s], node m at
+    // st_lo[s] + m st_step[s]; st_rows[s] == 0 for a switched-off stream.
+    const double* st_rq[kMaxStreams];
+    double st_step[kMaxStreams], st_lo[kMaxStreams];
+    int st_rows[kMaxStreams];
 };
 
 // Stars resident on the device, SoA.

@@ -327,7 +327,7 @@ __device__ __forceinline__ void row_by_recurrence(const double2* row, int n, dou
             const double2 e = row[i];
             bg_point<BG>(e.x, e.y, v.bg, K, acc[0]);
         }
-    } else if constexpr (Q > 1) {
+    } else {
         // (stream weight, background weight) entries; node i sits at r0 + i h
         // and only every Q-th node carries a background weight
         double rb = r0;
@@ -336,8 +336,6 @@ __device__ __forceinline__ void row_by_recurrence(const double2* row, int n, dou
             rb += Q * h;
         }
     }
-    // Q == 0: (r, weight) entries for the streams; the background runs on a
-    // grid of its own.
 #pragma unroll
     for (int s = 0; s < S; s++) {
         const double p = v.st[s].p;
@@ -358,7 +356,7 @@ __device__ __forceinline__ void row_by_recurrence(const double2* row, int n, dou
         const double c = C[s];
 #pragma unroll 4
         for (int i = 0; i < n; i++) {
-            a = fma(Q <= 1 ? row[i].y : row[i].x, g, a);
+            a = fma(Q == 1 ? row[i].y : row[i].x, g, a);
             g *= rr;
             rr *= c;
         }
@@ -370,9 +368,7 @@ __device__ __forceinline__ void row_by_recurrence(const double2* row, int n, dou
 
 // One logical block of a resampled region on a distance grid (grid_step > 0;
 // the region is evaluated whole, from tables in the recurrence layout): the
-// streams by recurrence, the background on every Q-th node, or (Q == 0) on
-// its own grid uniform in magnitude, whose bg_count (r, weight) entries pass
-// through the LDS tile first.  The resampled
+// streams by recurrence, the background on every Q-th node.  The resampled
 // tables carry every weight in the node entries (each row's irv and reff
 // are 1), so the column sums take the terms directly, plainly summed (a
 // column has a few hundred nodes, not the exact quadrature's tens of
@@ -409,20 +405,6 @@ __device__ __forceinline__ void integral_block_recurrence(const RegionLaunch& it
             const double lo = min_t2(fma(sight.st[s].p, it.grid_lo, -sight.st[s].m),
                                      fma(sight.st[s].p, r_last, -sight.st[s].m));
             floor_t2[s] = (float)(lo + kNegligibleLog2);   // rounding either way moves the bound by < 1e-5
-        }
-        if constexpr (Q == 0) {
-            const double2* bq = reinterpret_cast<const double2*>(it.bg_rq);
-            for (int e0 = 0; e0 < it.bg_count; e0 += kTileD2) {
-                const int m = (it.bg_count - e0 < kTileD2) ? it.bg_count - e0 : kTileD2;
-                __syncthreads();
-                for (int e = threadIdx.x; e < m; e += kBlock) tile[e] = bq[e0 + e];
-                __syncthreads();
-#pragma unroll 2
-                for (int i = 0; i < m; i++) {
-                    const double2 e = tile[i];
-                    bg_point<BG>(e.x, e.y, sight.bg, K, acc[0]);
-                }
-            }
         }
         const int rt = kTileD2 / n;
         const double2* rq = reinterpret_cast<const double2*>(it.rq);
@@ -464,6 +446,106 @@ __device__ __forceinline__ void integral_block_recurrence(const RegionLaunch& it
     }
 }
 
+// One logical block of a resampled region in layout 4 (grid_step > 0,
+// bg_stride 0): the background on its own grid uniform in magnitude (the
+// bg_count (r, weight) entries of bg_rq), then each switched-on stream by
+// recurrence on a distance grid of its own (st_rows rows of kResampleRow
+// nodes from st_rq, node m at st_lo + m st_step), with the row skipping and
+// restarts of row_by_recurrence.  Every table passes through the LDS tile.
+template <int S, int BG>
+__device__ __forceinline__ void integral_block_split(const RegionLaunch& it, long lb,
+                                                     const EvalConst* __restrict__ consts,
+                                                     double* __restrict__ prow, double2* tile,
+                                                     DD (*red)[kIntComps])
+{
+    const EvalConst& K = consts[it.slot];
+    const long C = it.n_cols;
+    const long col = lb * kBlock + threadIdx.x;
+    const bool valid = col < C;
+    const long cc = valid ? col : C - 1;
+    constexpr int n = kResampleRow;
+    constexpr int rt = kTileD2 / n;
+
+    double acc[1 + S];
+#pragma unroll
+    for (int c = 0; c <= S; c++) acc[c] = 0.0;
+    {
+        const Sight<S> sight = make_sight<S, BG>(it.col[cc], it.col[C + cc], it.col[2 * C + cc], K);
+        const double2* bq = reinterpret_cast<const double2*>(it.bg_rq);
+        for (int e0 = 0; e0 < it.bg_count; e0 += kTileD2) {
+            const int m = (it.bg_count - e0 < kTileD2) ? it.bg_count - e0 : kTileD2;
+            __syncthreads();
+            for (int e = threadIdx.x; e < m; e += kBlock) tile[e] = bq[e0 + e];
+            __syncthreads();
+#pragma unroll 2
+            for (int i = 0; i < m; i++) {
+                const double2 e = tile[i];
+                bg_point<BG>(e.x, e.y, sight.bg, K, acc[0]);
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            const int R = it.st_rows[s];
+            if (R == 0) continue;   // switched off: no terms
+            const double h = it.st_step[s], lo = it.st_lo[s];
+            const double p = sight.st[s].p, mm = sight.st[s].m;
+            const double d = p * h;
+            const double c = exp2_neg(2.0 * d * d);
+            const double floor_t2 =
+                min_t2(fma(p, lo, -mm), fma(p, fma((double)R * n - 1.0, h, lo), -mm)) + kNegligibleLog2;
+            const double2* rq = reinterpret_cast<const double2*>(it.st_rq[s]);
+            double G = 0.0, Rr = 0.0, a = 0.0;
+            for (int k0 = 0; k0 < R; k0 += rt) {
+                const int kt = (R - k0 < rt) ? R - k0 : rt;
+                __syncthreads();
+                for (int e = threadIdx.x; e < kt * n; e += kBlock) tile[e] = rq[(long)k0 * n + e];
+                __syncthreads();
+                for (int kk = 0; kk < kt; kk++) {
+                    const int k = k0 + kk;
+                    const double t0 = fma(p, fma((double)k * n, h, lo), -mm);
+                    const double t1 = fma(d, (double)(n - 1), t0);
+                    if (__all(min_t2(t0, t1) > floor_t2)) {
+                        G = 0.0;   // negligible here; a later row before the peak restarts
+                        continue;
+                    }
+                    if (k % kRestartRows == 0 || (t0 < 0.0 && G < 0x1p-1000)) {
+                        const double tc = fmax(t0, -40.0);
+                        G = exp2_neg(tc * tc);
+                        Rr = exp2_neg(fma(2.0 * tc, d, d * d));
+                    }
+                    const double2* row = tile + kk * n;
+#pragma unroll 4
+                    for (int i = 0; i < n; i++) {
+                        a = fma(row[i].y, G, a);
+                        G *= Rr;
+                        Rr *= c;
+                    }
+                }
+            }
+            acc[1 + s] = a;
+        }
+    }
+    DD out[kIntComps];
+#pragma unroll
+    for (int c = 0; c < kIntComps; c++) out[c] = DD{ 0.0, 0.0 };
+    if (valid) {
+        const double wnu = it.col[3 * C + cc];
+        const Sight<S> again = make_sight<S, BG>(it.col[cc], it.col[C + cc], it.col[2 * C + cc], K);
+        out[0] = DD{ acc[0] * wnu, 0.0 };
+#pragma unroll
+        for (int s = 0; s < S; s++)
+            out[1 + s] = DD{ acc[1 + s] * wnu * K.on[s] * exp2_neg(again.st[s].e), 0.0 };
+    }
+    block_reduce_dd<kIntComps>(out, red);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int c = 0; c < kIntComps; c++) {
+            prow[2 * c] = out[c].hi;
+            prow[2 * c + 1] = out[c].lo;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Integral: one logical block = 256 consecutive columns of one region of one
 // evaluation slot; the block's double-double partials go to prow.
@@ -476,7 +558,7 @@ __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
 {
     if constexpr (MIXED) {
         if (it.grid_step > 0.0) {
-            if (it.bg_stride == 0) integral_block_recurrence<S, BG, 0>(it, lb, consts, prow, tile, red);
+            if (it.bg_stride == 0) integral_block_split<S, BG>(it, lb, consts, prow, tile, red);
             else if (it.bg_stride == 4) integral_block_recurrence<S, BG, 4>(it, lb, consts, prow, tile, red);
             else if (it.bg_stride == 2) integral_block_recurrence<S, BG, 2>(it, lb, consts, prow, tile, red);
             else integral_block_recurrence<S, BG, 1>(it, lb, consts, prow, tile, red);

@@ -266,22 +266,30 @@ GridPlan plan_grid(const RegionDev& rd, const EvalConst& k, bool resample)
     } else {
         cost = table_nodes(rd, g.level, 1) * (kCostBg + kCostExpStream * S);
     }
-    // The split: a distance grid spaced for the streams alone, and the
-    // background grid.
-    double sigma = INFINITY;
+    // The split: each switched-on stream on a distance grid spaced for it
+    // alone (0.3 sigma_s; the recurrence's step k_s h is then 0.26 at most),
+    // and the background grid.
+    const int bl = bg_grid_level(rd, k.q);
+    if (bl < 0) return g;
+    GridPlan split;
+    split.layout = kLayoutSplit;
+    split.bg_level = bl;
+    double cost_split = bg_table_nodes(rd, bl) * kCostBg;
+    int finest = 0;
     for (int s = 0; s < k.n_streams; s++) {
         if (k.on[s] == 0.0 || !(k.inv_2s2[s] > 0.0)) continue;
-        sigma = std::min(sigma, std::sqrt(0.5 / k.inv_2s2[s]));
+        const double sigma = std::sqrt(0.5 / k.inv_2s2[s]);
+        const int L = level_for(0.3 * sigma, kDeltaKpc);
+        if (L >= kQuadLevels || !(k.ks[s] * table_delta(kQuadLevels + L) <= 0.5)) return g;
+        split.st_level[s] = L;
+        finest = std::max(finest, L);
+        cost_split += table_nodes(rd, kQuadLevels + L, 1) * (kCostNode + kCostRecStream);
     }
-    const int Ls = level_for(0.3 * sigma, kDeltaKpc);
-    const int bl = bg_grid_level(rd, k.q);
-    if (Ls >= kQuadLevels || bl < 0) return g;
-    const int level = kQuadLevels + Ls;
-    if (!(recurrence_step(level, k) > 0.0)) return g;
-    const double split = table_nodes(rd, level, 1) * (kCostNode + kCostRecStream * S) +
-                         bg_table_nodes(rd, bl) * kCostBg;
-    if (split < 0.9 * cost) g = GridPlan{ level, kLayoutSplit, bl };
-    return g;
+    // The plan's own table: the finest stream grid (the 2 kpc grid with no
+    // stream on), which the launch's node count refers to.
+    split.level = kQuadLevels + finest;
+    if (!(recurrence_step(split.level, k) > 0.0)) return g;
+    return cost_split < 0.9 * cost ? split : g;
 }
 
 static int upload_table(const RegionTables& t, long M, std::unique_ptr<ResampledDev>* slot)
@@ -327,6 +335,25 @@ int region_tables(RegionDev& rd, const GridPlan& g, TableRef* out)
     *out = TableRef{ q.rq.p, q.rstep.p, q.r_steps, q.n_conv, level, q.nodes, layout, q.lo,
                      stride > 1 ? (q.nodes + stride - 1) / stride : q.nodes };
     if (layout == kLayoutSplit) {
+        for (int s = 0; s < kMaxStreams; s++) {
+            if (g.st_level[s] < 0) continue;
+            const int ls = kQuadLevels + g.st_level[s];
+            if (ls >= kQuadTables) return fail(MW_E_INVALID, "no such quadrature table");
+            std::unique_ptr<ResampledDev>& sslot = rd.levels[ls][0];
+            if (!sslot) {
+                RegionTables t;
+                const long M = build_resampled_tables(rd.h_rq, rd.h_rstep, rd.n_conv, table_delta(ls), true, 0, &t);
+                if (M < 1) return fail(MW_E_INVALID, "resampled quadrature failed");
+                int rc;
+                if ((rc = upload_table(t, M, &sslot))) return rc;
+            }
+            if (sslot->n_conv != kResampleRow) return fail(MW_E_INVALID, "stream table rows");
+            out->st_rq[s] = sslot->rq.p;
+            out->st_step[s] = table_delta(ls);
+            out->st_lo[s] = sslot->lo;
+            out->st_rows[s] = sslot->r_steps;
+            out->stream_terms += (long)sslot->r_steps * sslot->n_conv;
+        }
         std::unique_ptr<ResampledDev>& bslot = rd.bg_levels[g.bg_level];
         if (!bslot) {
             RegionTables t;

@@ -80,7 +80,7 @@ class mw_timing(C.Structure):
     _fields_ = [("integral_ms", C.c_double), ("star_ms", C.c_double), ("total_ms", C.c_double),
                 ("integral_blocks", C.c_long), ("star_blocks", C.c_long),
                 ("integral_points", C.c_long), ("integral_points_recurrence", C.c_long),
-                ("integral_points_background", C.c_long)]
+                ("integral_points_background", C.c_long), ("integral_stream_terms", C.c_long)]
 
 
 class mw_server_config(C.Structure):

@@ -156,6 +156,35 @@ def test_background_grid():
     assert lnp <= LNP_TOL
 
 
+def test_stream_grids():
+    """With the background on its own grid each switched-on stream runs on a
+    distance grid spaced for it alone: a 4 kpc stream beside a 1 kpc one
+    costs fewer terms than the shared grid (nodes x streams), a switched-off
+    stream none, and the results still match the oracle."""
+    p, stars = synth.workload(95, LONG_R, 3)
+    for st, sigma in zip(p.streams, (1.0, 4.0, 4.0)):
+        st.parameters[4] = sigma
+    ph, sh = ParamsHolder(p), StarsHolder(stars)
+    vec = p.parameter_vector()
+    with sep.Context(ph, sh, timing=True) as ctx:
+        got = ctx.eval(vec, precision=sep.PREC_RESAMPLED, per_star=True)[0]
+        tm = ctx.timing()
+        off = vec.copy()
+        off[2 + 2 * 6 + 5] = 1e-5    # third stream's sigma: switched off
+        ctx.eval(off, precision=sep.PREC_RESAMPLED)
+        tm_off = ctx.timing()
+    ref = O.evaluate_full(ParamsHolder(p), StarsHolder(stars))
+    e_st = rel(got.stream_integrals, ref.stream_integrals)
+    lnp = math.log(10) * np.max(np.abs(got.log10_p - ref.log10_p))
+    shared = tm.integral_points_recurrence * 3
+    print(f"stream terms {tm.integral_stream_terms} (shared grid {shared}), with one off "
+          f"{tm_off.integral_stream_terms}; I_st {e_st:.2e} lnP {lnp:.2e}")
+    assert tm.integral_points_recurrence == tm.integral_points
+    assert tm.integral_stream_terms < 0.7 * shared
+    assert tm_off.integral_stream_terms < tm.integral_stream_terms
+    assert e_st <= 1e-9 and lnp <= LNP_TOL
+
+
 def test_background_stride(monkeypatch):
     """With the background's own grid switched off (MW_HIP_BG_GRID=0) it is
     sampled on every 2nd or 4th node of the distance grid where it is wider
