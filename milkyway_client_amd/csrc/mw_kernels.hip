@@ -84,6 +84,50 @@ __device__ __forceinline__ void block_reduce_dd(DD (&v)[NC], DD (*lds)[NC])
     }
 }
 
+// Sum one double per work-item over the workgroup in a fixed order (an xor
+// butterfly within each wave, whose partners add the same two values, then
+// the waves in order); result valid in thread 0.  The mixed kernels' block
+// partials: a few hundred positive terms per block, summed with a relative
+// error of a few ulp, the blocks then added in double-double by
+// reduce_kernel.
+template <int NC>
+__device__ __forceinline__ void block_reduce_f64(double (&v)[NC], double (*lds)[NC])
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) v[c] += __shfl_xor(v[c], m, 64);
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) lds[wave][c] = v[c];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            double a = lds[0][c];
+            for (int w = 1; w < kBlock / 64; w++) a += lds[w][c];
+            v[c] = a;
+        }
+    }
+}
+
+// The mixed integral's block result: components 0..S summed as above into
+// prow's double-double pairs (lo 0), the unused stream components 0.
+template <int S>
+__device__ __forceinline__ void write_block_f64(double (&out)[1 + S], double* __restrict__ prow, void* red)
+{
+    block_reduce_f64<1 + S>(out, reinterpret_cast<double(*)[1 + S]>(red));
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int c = 0; c < 2 * kIntComps; c++) prow[c] = 0.0;
+#pragma unroll
+        for (int c = 0; c <= S; c++) prow[2 * c] = out[c];
+    }
+}
+
 __device__ __forceinline__ void kahan_add(double& s, double& c, double x)
 {
     const double t = s;
@@ -425,25 +469,17 @@ __device__ __forceinline__ void integral_block_recurrence(const RegionLaunch& it
             }
         }
     }
-    DD out[kIntComps];
+    double out[1 + S];
 #pragma unroll
-    for (int c = 0; c < kIntComps; c++) out[c] = DD{ 0.0, 0.0 };
+    for (int c = 0; c <= S; c++) out[c] = 0.0;
     if (valid) {
         const double wnu = it.col[3 * C + cc];
         const Sight<S> again = make_sight<S, BG>(it.col[cc], it.col[C + cc], it.col[2 * C + cc], K);
-        out[0] = DD{ acc[0] * wnu, 0.0 };
+        out[0] = acc[0] * wnu;
 #pragma unroll
-        for (int s = 0; s < S; s++)
-            out[1 + s] = DD{ acc[1 + s] * wnu * K.on[s] * exp2_neg(again.st[s].e), 0.0 };
+        for (int s = 0; s < S; s++) out[1 + s] = acc[1 + s] * wnu * K.on[s] * exp2_neg(again.st[s].e);
     }
-    block_reduce_dd<kIntComps>(out, red);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int c = 0; c < kIntComps; c++) {
-            prow[2 * c] = out[c].hi;
-            prow[2 * c + 1] = out[c].lo;
-        }
-    }
+    write_block_f64<S>(out, prow, red);
 }
 
 // One logical block of a resampled region in layout 4 (grid_step > 0,
@@ -525,25 +561,17 @@ __device__ __forceinline__ void integral_block_split(const RegionLaunch& it, lon
             acc[1 + s] = a;
         }
     }
-    DD out[kIntComps];
+    double out[1 + S];
 #pragma unroll
-    for (int c = 0; c < kIntComps; c++) out[c] = DD{ 0.0, 0.0 };
+    for (int c = 0; c <= S; c++) out[c] = 0.0;
     if (valid) {
         const double wnu = it.col[3 * C + cc];
         const Sight<S> again = make_sight<S, BG>(it.col[cc], it.col[C + cc], it.col[2 * C + cc], K);
-        out[0] = DD{ acc[0] * wnu, 0.0 };
+        out[0] = acc[0] * wnu;
 #pragma unroll
-        for (int s = 0; s < S; s++)
-            out[1 + s] = DD{ acc[1 + s] * wnu * K.on[s] * exp2_neg(again.st[s].e), 0.0 };
+        for (int s = 0; s < S; s++) out[1 + s] = acc[1 + s] * wnu * K.on[s] * exp2_neg(again.st[s].e);
     }
-    block_reduce_dd<kIntComps>(out, red);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int c = 0; c < kIntComps; c++) {
-            prow[2 * c] = out[c].hi;
-            prow[2 * c + 1] = out[c].lo;
-        }
-    }
+    write_block_f64<S>(out, prow, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -798,12 +826,28 @@ __device__ __forceinline__ void star_block(const StarLaunch& it, long lb,
     } else if (log10p) {
         log10p[(long)it.out_index * log10p_stride + idx] = lp;
     }
-    block_reduce_dd<kStarComps>(out, red);
-    if (threadIdx.x == 0) {
+    if constexpr (MIXED) {
+        // one double per component (every lo is 0 here), summed as the
+        // mixed integral's partials
+        double v[kStarComps];
 #pragma unroll
-        for (int c = 0; c < kStarComps; c++) {
-            prow[2 * c] = out[c].hi;
-            prow[2 * c + 1] = out[c].lo;
+        for (int c = 0; c < kStarComps; c++) v[c] = out[c].hi;
+        block_reduce_f64<kStarComps>(v, reinterpret_cast<double(*)[kStarComps]>(red));
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int c = 0; c < kStarComps; c++) {
+                prow[2 * c] = v[c];
+                prow[2 * c + 1] = 0.0;
+            }
+        }
+    } else {
+        block_reduce_dd<kStarComps>(out, red);
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int c = 0; c < kStarComps; c++) {
+                prow[2 * c] = out[c].hi;
+                prow[2 * c + 1] = out[c].lo;
+            }
         }
     }
 }
