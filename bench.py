@@ -323,10 +323,10 @@ def bench_batch(args):
 
     quad = "resampled" if head_prec == sep.PREC_RESAMPLED else "exact"
     quad_desc = ("resampled: each region's r-step x convolution weights folded by 16-node Lagrange "
-                 "interpolation onto a grid in distance or magnitude sized to the model's narrowest "
-                 "feature (MW_PREC_RESAMPLED); on distance grids the stream Gaussians by FP64 "
-                 "recurrence (rows a stream does not reach skipped) and the background on every 1st, "
-                 "2nd or 4th node as its width allows; star pass exact" if quad == "resampled" else
+                 "interpolation onto grids sized to the model's features (MW_PREC_RESAMPLED): each "
+                 "stream's Gaussian by FP64 recurrence on a distance grid of its own (rows it does "
+                 "not reach skipped), the background on a magnitude grid of its own; star pass "
+                 "exact" if quad == "resampled" else
                  "exact: the reference's r-step x convolution points (MW_PREC_MIXED)")
     out = {
         "metric": "likelihood evals/s, batched mixed precision vs plain FP64 @ per-star |d lnP| <= 1e-7",
