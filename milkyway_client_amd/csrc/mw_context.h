@@ -255,7 +255,7 @@ constexpr int kQuadTables = 2 * kQuadLevels;
 // the background on every 1st, 2nd, 4th node; 4 = the recurrence on a
 // distance grid for the streams alone, the background on a grid of its own
 // uniform in magnitude (a background table, index t < kBgLevels: spacing
-// 0.25 * 2^(-t/2) mag).
+// 0.3 * 2^(-t/2) mag).
 constexpr int kTableLayouts = 5;
 constexpr int kLayoutSplit = 4;
 constexpr int kBgLevels = 13;
@@ -313,7 +313,7 @@ struct TableRef {
     double lo;         // resampled: the grid's first node
     long bg_nodes;     // background evaluations per sight (nodes / stride in the recurrence layouts)
     const double* bg_rq = nullptr;   // layout 4: the background grid's (r, weight) entries
-    long bg_count = 0;               // and their number (rows of kResampleRow, padding weightless)
+    long bg_count = 0;               // and their number (the fold's lattice)
     long stream_terms = 0;           // layout 4: stream terms per sight (the streams' grids' nodes)
     // layout 4: each stream's own distance table (st_rows 0: switched off)
     const double* st_rq[kMaxStreams] = {};
@@ -340,7 +340,7 @@ double bg_width(const RegionDev& rd, double q);
 // A lower bound on that angle over the region's sights for flattening q.
 double bg_theta(const RegionDev& rd, double q);
 // Background table index for flattening q (the coarsest spacing within
-// 0.15 x 2.17 x theta and 0.25 mag), or -1 if none is fine enough.
+// 0.2 x 2.17 x theta and 0.3 mag), or -1 if none is fine enough.
 int bg_grid_level(const RegionDev& rd, double q);
 
 // Resampled table index to use for region rd under the model k, or -1 for

@@ -36,7 +36,7 @@ namespace {
 constexpr double kMagPerLn = 2.1714724095162588;   // 5 / ln 10
 constexpr double kDeltaMag = 0.02;                  // coarsest magnitude spacing
 constexpr double kDeltaKpc = 2.0;                   // coarsest distance spacing
-constexpr double kDeltaBgMag = 0.25;                // coarsest background-grid spacing
+constexpr double kDeltaBgMag = 0.3;                 // coarsest background-grid spacing
 
 // Spacing of table index t (see kQuadLevels).
 double table_delta(int t)
@@ -143,14 +143,15 @@ double bg_theta(const RegionDev& rd, double q)
 
 int bg_grid_level(const RegionDev& rd, double q)
 {
-    // 16-node stencils on a grid uniform in x = 2.17 ln r: spacing 0.15 of
+    // 16-node stencils on a grid uniform in x = 2.17 ln r: spacing 0.2 of
     // the singularities' distance from the real axis (2.17 theta), and at
-    // most 0.25 mag, where the profile's r^-4 tail (an exponential of rate
+    // most 0.3 mag, where the profile's r^-4 tail (an exponential of rate
     // 1.84 per mag in x) sets the interpolation error instead.  Measured on
     // the fold of a dense quadrature over 1-105 kpc, every (rg_min, P, r0)
     // from (0.5-8.4 kpc, 0.5-5, 1-14 kpc), towards and away from the
-    // centre: relative error of the integral below 5.2e-12.
-    const double dg = std::min(0.15 * kMagPerLn * bg_theta(rd, q), kDeltaBgMag);
+    // centre: relative error of the integral below 1.1e-10 (4.6e-11 at
+    // 0.15 of the distance, 5.2e-12 at 0.15 and 0.25 mag, 5e-10 at 0.35 mag).
+    const double dg = std::min(0.2 * kMagPerLn * bg_theta(rd, q), kDeltaBgMag);
     if (!(dg > 0.0)) return -1;
     const int t = level_for(dg, kDeltaBgMag);
     return t < kBgLevels ? t : -1;
@@ -241,11 +242,13 @@ long table_nodes(const RegionDev& rd, int level, int stride)
     return padded_rows(M);
 }
 
+// The background grid's nodes (the kernel runs over these, not the row
+// padding).
 long bg_table_nodes(const RegionDev& rd, int bg_level)
 {
     const int half = kResampleDegree / 2;
     const double d = kDeltaBgMag * std::pow(2.0, -0.5 * bg_level);
-    return padded_rows((long)std::ceil(rd.g_span / d) + 2 * half + 3);
+    return (long)std::ceil(rd.g_span / d) + 2 * half + 3;
 }
 
 }  // namespace
@@ -364,7 +367,7 @@ int region_tables(RegionDev& rd, const GridPlan& g, TableRef* out)
             if ((rc = upload_table(t, M, &bslot))) return rc;
         }
         out->bg_rq = bslot->rq.p;
-        out->bg_count = (long)bslot->r_steps * bslot->n_conv;
+        out->bg_count = bslot->nodes;   // the fold's lattice; the row padding carries no weight
         out->bg_nodes = out->bg_count;
     }
     return MW_OK;
