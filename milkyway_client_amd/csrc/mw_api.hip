@@ -489,9 +489,9 @@ size_t mw_ctx_resident_bytes(const mw_ctx* c)
         std::lock_guard<std::mutex> lock(r->level_mu);
         for (const auto& per_level : r->levels)
             for (const auto& q : per_level)
-                if (q) b += q->rq.cap * sizeof(double) + q->rstep.cap * sizeof(RStep);
+                if (q) b += (q->rq.cap + q->w.cap) * sizeof(double) + q->rstep.cap * sizeof(RStep);
         for (const auto& q : r->bg_levels)
-            if (q) b += q->rq.cap * sizeof(double) + q->rstep.cap * sizeof(RStep);
+            if (q) b += (q->rq.cap + q->w.cap) * sizeof(double) + q->rstep.cap * sizeof(RStep);
     }
     b += (c->star.cap + c->ratio.cap + c->kw.cap) * sizeof(double);
     return b;

@@ -269,6 +269,7 @@ struct ResampledDev {
     double lo = 0.0;
     DevBuf<double> rq;
     DevBuf<RStep> rstep;
+    DevBuf<double> w;   // (r, weight) tables: the weights alone, contiguous (layout 4's streams)
 };
 
 struct RegionDev {
@@ -316,7 +317,7 @@ struct TableRef {
     long bg_count = 0;               // and their number (the fold's lattice)
     long stream_terms = 0;           // layout 4: stream terms per sight (the streams' grids' nodes)
     // layout 4: each stream's own distance table (st_rows 0: switched off)
-    const double* st_rq[kMaxStreams] = {};
+    const double* st_rq[kMaxStreams] = {};   // the weights alone
     double st_step[kMaxStreams] = {}, st_lo[kMaxStreams] = {};
     int st_rows[kMaxStreams] = {};
 };

@@ -91,7 +91,7 @@ struct RegionLaunch {
     int bg_count;
     const double* bg_rq;
     // bg_stride 0: each stream on a distance grid of its own, st_rows rows of
-    // kResampleRow (r, weight) entries from st_rq[s], node m at
+    // kResampleRow weights from st_rq[s] (the weights alone), node m at
     // st_lo[s] + m st_step[s]; st_rows[s] == 0 for a switched-off stream.
     const double* st_rq[kMaxStreams];
     double st_step[kMaxStreams], st_lo[kMaxStreams];

@@ -486,7 +486,7 @@ __device__ __forceinline__ void integral_block_recurrence(const RegionLaunch& it
 // bg_stride 0): the background on its own grid uniform in magnitude (the
 // bg_count (r, weight) entries of bg_rq), then each switched-on stream by
 // recurrence on a distance grid of its own (st_rows rows of kResampleRow
-// nodes from st_rq, node m at st_lo + m st_step), with the row skipping and
+// weights from st_rq, node m at st_lo + m st_step), with the row skipping and
 // restarts of row_by_recurrence.  Every table entry is the same for all
 // lanes, so the tables are read through the constant address space, where
 // uniform loads become scalar loads into SGPRs: no LDS staging and no
@@ -537,10 +537,10 @@ __device__ __forceinline__ void integral_block_split(const RegionLaunch& it, lon
                     G = exp2_neg(tc * tc);
                     Rr = exp2_neg(fma(2.0 * tc, d, d * d));
                 }
-                const ConstPtr row = wq + 2 * (long)k * n + 1;   // weights of (r, weight) pairs
+                const ConstPtr row = wq + (long)k * n;
 #pragma unroll 4
                 for (int i = 0; i < n; i++) {
-                    a = fma(row[2 * i], G, a);
+                    a = fma(row[i], G, a);
                     G *= Rr;
                     Rr *= c;
                 }

@@ -295,7 +295,9 @@ GridPlan plan_grid(const RegionDev& rd, const EvalConst& k, bool resample)
     return cost_split < 0.9 * cost ? split : g;
 }
 
-static int upload_table(const RegionTables& t, long M, std::unique_ptr<ResampledDev>* slot)
+// pairs: the table holds (r, weight) pairs, whose weights alone are also
+// uploaded, contiguous.
+static int upload_table(const RegionTables& t, long M, bool pairs, std::unique_ptr<ResampledDev>* slot)
 {
     auto q = std::make_unique<ResampledDev>();
     q->r_steps = t.r_steps;
@@ -306,6 +308,12 @@ static int upload_table(const RegionTables& t, long M, std::unique_ptr<Resampled
     if ((rc = q->rq.reserve(t.rq.size())) || (rc = q->rstep.reserve(t.rstep.size()))) return rc;
     MW_HIP_CHECK(hipMemcpy(q->rq.p, t.rq.data(), sizeof(double) * t.rq.size(), hipMemcpyHostToDevice));
     MW_HIP_CHECK(hipMemcpy(q->rstep.p, t.rstep.data(), sizeof(RStep) * t.rstep.size(), hipMemcpyHostToDevice));
+    if (pairs) {
+        std::vector<double> w(t.rq.size() / 2);
+        for (size_t i = 0; i < w.size(); i++) w[i] = t.rq[2 * i + 1];
+        if ((rc = q->w.reserve(w.size()))) return rc;
+        MW_HIP_CHECK(hipMemcpy(q->w.p, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice));
+    }
     *slot = std::move(q);
     return MW_OK;
 }
@@ -332,7 +340,7 @@ int region_tables(RegionDev& rd, const GridPlan& g, TableRef* out)
                                               level >= kQuadLevels, stride > 1 ? stride : 0, &t);
         if (M < 1) return fail(MW_E_INVALID, "resampled quadrature failed");
         int rc;
-        if ((rc = upload_table(t, M, &slot))) return rc;
+        if ((rc = upload_table(t, M, stride <= 1, &slot))) return rc;
     }
     const ResampledDev& q = *slot;
     *out = TableRef{ q.rq.p, q.rstep.p, q.r_steps, q.n_conv, level, q.nodes, layout, q.lo,
@@ -348,10 +356,10 @@ int region_tables(RegionDev& rd, const GridPlan& g, TableRef* out)
                 const long M = build_resampled_tables(rd.h_rq, rd.h_rstep, rd.n_conv, table_delta(ls), true, 0, &t);
                 if (M < 1) return fail(MW_E_INVALID, "resampled quadrature failed");
                 int rc;
-                if ((rc = upload_table(t, M, &sslot))) return rc;
+                if ((rc = upload_table(t, M, true, &sslot))) return rc;
             }
-            if (sslot->n_conv != kResampleRow) return fail(MW_E_INVALID, "stream table rows");
-            out->st_rq[s] = sslot->rq.p;
+            if (sslot->n_conv != kResampleRow || !sslot->w.p) return fail(MW_E_INVALID, "stream table rows");
+            out->st_rq[s] = sslot->w.p;
             out->st_step[s] = table_delta(ls);
             out->st_lo[s] = sslot->lo;
             out->st_rows[s] = sslot->r_steps;
@@ -364,7 +372,7 @@ int region_tables(RegionDev& rd, const GridPlan& g, TableRef* out)
                                                   kDeltaBgMag * std::pow(2.0, -0.5 * g.bg_level), false, 0, &t);
             if (M < 1) return fail(MW_E_INVALID, "resampled quadrature failed");
             int rc;
-            if ((rc = upload_table(t, M, &bslot))) return rc;
+            if ((rc = upload_table(t, M, true, &bslot))) return rc;
         }
         out->bg_rq = bslot->rq.p;
         out->bg_count = bslot->nodes;   // the fold's lattice; the row padding carries no weight
