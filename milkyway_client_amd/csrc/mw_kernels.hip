@@ -746,9 +746,13 @@ __device__ __forceinline__ void star_densities(const double* __restrict__ star, 
     bg = 0.0;
 #pragma unroll
     for (int s = 0; s < kMaxStreams; s++) st[s] = 0.0;
+    // The convolution tables are the same for every lane: read through the
+    // constant address space, where uniform loads become scalar loads.
+    typedef const __attribute__((address_space(4))) double* ConstPtr;
+    const ConstPtr ratio = (ConstPtr)conv_ratio, kw = (ConstPtr)conv_kw;
     for (int j = 0; j < n; j++) {
-        const double r = rstar * conv_ratio[j];
-        const double qw = conv_kw[j] * (r * r * r);
+        const double r = rstar * ratio[j];
+        const double qw = kw[j] * (r * r * r);
         point_terms<S, BG, MIXED>(r, qw, sight, K, bg, st);
     }
     if constexpr (BG == kBgZeroQ) {
