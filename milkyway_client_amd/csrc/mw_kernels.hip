@@ -226,15 +226,24 @@ struct Sight {
     Quad st[S > 0 ? S : 1];
 };
 
-template <int S, int BG>
+// FAST (mixed kernels): the square roots and quotients from one
+// Newton-refined reciprocal square root per quadratic (rsqrt_nr, 4e-15
+// relative) instead of an IEEE square root and divisions; the FP64 kernels
+// keep the correctly rounded operations.
+template <int S, int BG, bool FAST = false>
 __device__ __forceinline__ Sight<S> make_sight(double ux, double uy, double uz, const EvalConst& K)
 {
     Sight<S> v;
     if constexpr (BG != kBgZeroQ) {
         const double side = fma(uy, uy, uz * uz * K.inv_q2);
         const double P = fma(ux, ux, side);
-        const double p = sqrt(P);
-        v.bg = Quad{ p, kSunR0 * ux / p, (kSunR0 * kSunR0) * side / P };
+        if constexpr (FAST) {
+            const double ip = rsqrt_nr(P);
+            v.bg = Quad{ P * ip, kSunR0 * ux * ip, (kSunR0 * kSunR0) * side * (ip * ip) };
+        } else {
+            const double p = sqrt(P);
+            v.bg = Quad{ p, kSunR0 * ux / p, (kSunR0 * kSunR0) * side / P };
+        }
     } else {
         v.bg = Quad{ 0.0, 0.0, 0.0 };
     }
@@ -247,8 +256,15 @@ __device__ __forceinline__ Sight<S> make_sight(double ux, double uy, double uz, 
         const double ww = fma(w0, w0, fma(w1, w1, w2 * w2));
         const double c0 = K.cperp[s][0], c1 = K.cperp[s][1], c2 = K.cperp[s][2];
         if (ww > 1e-200) {
-            const double p = sqrt(ww);
-            const double h0 = w0 / p, h1 = w1 / p, h2 = w2 / p;
+            double p, h0, h1, h2;
+            if constexpr (FAST) {
+                const double ip = rsqrt_nr(ww);
+                p = ww * ip;
+                h0 = w0 * ip, h1 = w1 * ip, h2 = w2 * ip;
+            } else {
+                p = sqrt(ww);
+                h0 = w0 / p, h1 = w1 / p, h2 = w2 / p;
+            }
             const double m = fma(h0, c0, fma(h1, c1, h2 * c2));
             const double e0 = fma(-m, h0, c0), e1 = fma(-m, h1, c1), e2 = fma(-m, h2, c2);
             v.st[s] = Quad{ p, m, fma(e0, e0, fma(e1, e1, e2 * e2)) };
@@ -437,7 +453,7 @@ __device__ __forceinline__ void integral_block_recurrence(const RegionLaunch& it
 #pragma unroll
     for (int c = 0; c <= S; c++) acc[c] = 0.0;
     {
-        const Sight<S> sight = make_sight<S, BG>(it.col[cc], it.col[C + cc], it.col[2 * C + cc], K);
+        const Sight<S> sight = make_sight<S, BG, true>(it.col[cc], it.col[C + cc], it.col[2 * C + cc], K);
         double Cs[S > 0 ? S : 1], G[S > 0 ? S : 1], Rr[S > 0 ? S : 1];
         float floor_t2[S > 0 ? S : 1];   // rows whose t^2 all exceed this are negligible
         const double r_last = fma((double)R * n - 1.0, h, it.grid_lo);
@@ -474,7 +490,7 @@ __device__ __forceinline__ void integral_block_recurrence(const RegionLaunch& it
     for (int c = 0; c <= S; c++) out[c] = 0.0;
     if (valid) {
         const double wnu = it.col[3 * C + cc];
-        const Sight<S> again = make_sight<S, BG>(it.col[cc], it.col[C + cc], it.col[2 * C + cc], K);
+        const Sight<S> again = make_sight<S, BG, true>(it.col[cc], it.col[C + cc], it.col[2 * C + cc], K);
         out[0] = acc[0] * wnu;
 #pragma unroll
         for (int s = 0; s < S; s++) out[1 + s] = acc[1 + s] * wnu * K.on[s] * exp2_neg(again.st[s].e);
@@ -509,7 +525,7 @@ __device__ __forceinline__ void integral_block_split(const RegionLaunch& it, lon
 #pragma unroll
     for (int c = 0; c <= S; c++) acc[c] = 0.0;
     {
-        const Sight<S> sight = make_sight<S, BG>(it.col[cc], it.col[C + cc], it.col[2 * C + cc], K);
+        const Sight<S> sight = make_sight<S, BG, true>(it.col[cc], it.col[C + cc], it.col[2 * C + cc], K);
         const ConstPtr bq = (ConstPtr)(it.bg_rq);
 #pragma unroll 2
         for (int i = 0; i < it.bg_count; i++) bg_point<BG>(bq[2 * i], bq[2 * i + 1], sight.bg, K, acc[0]);
@@ -553,7 +569,7 @@ __device__ __forceinline__ void integral_block_split(const RegionLaunch& it, lon
     for (int c = 0; c <= S; c++) out[c] = 0.0;
     if (valid) {
         const double wnu = it.col[3 * C + cc];
-        const Sight<S> again = make_sight<S, BG>(it.col[cc], it.col[C + cc], it.col[2 * C + cc], K);
+        const Sight<S> again = make_sight<S, BG, true>(it.col[cc], it.col[C + cc], it.col[2 * C + cc], K);
         out[0] = acc[0] * wnu;
 #pragma unroll
         for (int s = 0; s < S; s++) out[1 + s] = acc[1 + s] * wnu * K.on[s] * exp2_neg(again.st[s].e);
@@ -596,7 +612,7 @@ __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
     const long last = (col0 + kBlock < C ? col0 + kBlock : C);
     const bool active = it.calc_end > col0 * R && it.calc_begin < last * R;
     if (active) {
-        const Sight<S> sight = make_sight<S, BG>(it.col[cc], it.col[C + cc], it.col[2 * C + cc], K);
+        const Sight<S> sight = make_sight<S, BG, MIXED>(it.col[cc], it.col[C + cc], it.col[2 * C + cc], K);
         const double wnu = it.col[3 * C + cc];
         const long base = cc * (long)R;
         long lo_l = it.calc_begin - base, hi_l = it.calc_end - base;
@@ -739,7 +755,7 @@ __device__ __forceinline__ void star_densities(const double* __restrict__ star, 
                                                const double* __restrict__ conv_kw, double& reff,
                                                double& bg, double (&st)[kMaxStreams])
 {
-    const Sight<S> sight = make_sight<S, BG>(star[i], star[N + i], star[2 * N + i], K);
+    const Sight<S> sight = make_sight<S, BG, MIXED>(star[i], star[N + i], star[2 * N + i], K);
     const double rstar = star[3 * N + i];
     reff = star[4 * N + i];
 
