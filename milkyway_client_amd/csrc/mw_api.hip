@@ -128,6 +128,15 @@ int build_context(mw_ctx* c, const std::vector<RegionGeom>& geoms, int n_conv, i
         MW_HIP_CHECK(hipMemcpy(c->star.p, soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice));
     }
     star_conv_tables(n_conv, &ratio, &kw);
+    // The resampled star pass folds the background over the convolution's
+    // magnitude offsets dx = 5 log10(ratio), weights kw ratio^3.
+    c->conv_dx.resize(n_conv);
+    c->conv_w3.resize(n_conv);
+    for (int i = 0; i < n_conv; i++) {
+        c->conv_dx[i] = 5.0 * std::log10(ratio[i]);
+        c->conv_w3[i] = kw[i] * (ratio[i] * ratio[i] * ratio[i]);
+    }
+    if (c->n_stars > 0) tabulate_star_theta(*c, soa.data(), c->n_stars);
     int rc;
     if ((rc = c->ratio.reserve(n_conv)) || (rc = c->kw.reserve(n_conv))) return rc;
     MW_HIP_CHECK(hipMemcpy(c->ratio.p, ratio.data(), sizeof(double) * n_conv, hipMemcpyHostToDevice));
@@ -258,6 +267,8 @@ int run_plan(mw_ctx* c, Plan& P)
             L.n_streams = c->S;
             L.ratio = c->ratio.p;
             L.kw = c->kw.p;
+            const int sl = (P.resample && P.mixed && bg == kBgHernquist) ? star_bg_level(*c, P.K[s].q) : -1;
+            if ((rc = star_bg_table(*c, sl, &L.bg_rq, &L.bg_count))) return rc;
             ReduceItem& R = c->h_sred.p[oi];
             R.block_begin = total_sblocks;
             R.block_count = blocks;

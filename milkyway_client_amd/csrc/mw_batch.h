@@ -43,6 +43,7 @@ public:
     void set_constants(const std::vector<Slot>& slots);
     bool same_modes(const std::vector<Slot>& slots) const;
     int table_choice(const RegionDev& rd, const EvalConst& k) const;
+    int star_level(const mw_ctx& c, const EvalConst& k) const;
     // Enqueues the whole sequence on the runner's stream (capturable).  With
     // kernel_events, events bracket the integral launches (not while a
     // graph is being captured).

@@ -45,6 +45,13 @@ int BatchRunner::table_choice(const RegionDev& rd, const EvalConst& k) const
     return key;
 }
 
+// The star pass's background table under the model k (resampled mixed
+// batches of the Hernquist profile), or -1 for the exact convolution.
+int BatchRunner::star_level(const mw_ctx& c, const EvalConst& k) const
+{
+    return (resample_ && mixed_ && k.bg_mode == kBgHernquist) ? star_bg_level(c, k.q) : -1;
+}
+
 bool BatchRunner::same_modes(const std::vector<Slot>& slots) const
 {
     if (slots.size() != slot_bg_.size()) return false;
@@ -56,6 +63,8 @@ bool BatchRunner::same_modes(const std::vector<Slot>& slots) const
         // which follow from the model.
         for (const auto& rd : slots[s].ctx->regions)
             if (li >= levels_.size() || table_choice(*rd, slots[s].k) != levels_[li++]) return false;
+        // and the star pass's background table
+        if (li >= levels_.size() || star_level(*slots[s].ctx, slots[s].k) != levels_[li++]) return false;
     }
     return li == levels_.size();
 }
@@ -152,6 +161,7 @@ int BatchRunner::plan(const std::vector<Slot>& slots, int precision)
                                                        : (double)L.r_steps * L.n_conv * (2.0 + L.n_streams);
             items.push_back(It{ L, cost });
         }
+        levels_.push_back(star_level(*c, slots[s].k));
     }
     n_ritems_ = (int)items.size();
     // Group by background mode, largest cost per block first within a group.
@@ -219,6 +229,8 @@ int BatchRunner::plan(const std::vector<Slot>& slots, int precision)
         L.n_streams = slots[s].k.n_streams;
         L.ratio = c->ratio.p;
         L.kw = c->kw.p;
+        int rc1;
+        if ((rc1 = star_bg_table(*c, star_level(*c, slots[s].k), &L.bg_rq, &L.bg_count))) return rc1;
         hs[oi] = L;
         hsr[oi] = ReduceItem{ total_sblocks_, blocks, s, 0 };
         g.n_items++;

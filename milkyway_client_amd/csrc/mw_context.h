@@ -391,6 +391,14 @@ struct mw_ctx {
     std::vector<std::unique_ptr<mw::RegionDev>> regions;
     long n_stars = 0;
     mw::DevBuf<double> star, ratio, kw;
+    // Resampled star pass: the convolution points' magnitude offsets and
+    // weights kw ratio^3 (host), the smallest sin^2 theta over the stars at
+    // the tabulated q values (see star_bg_level), and the background tables
+    // folded onto grids uniform in the offset, built on first use.
+    std::vector<double> conv_dx, conv_w3;
+    std::vector<double> star_sin2_q;
+    std::unique_ptr<mw::ResampledDev> star_bg[mw::kBgLevels];
+    std::mutex star_bg_mu;
 
     // per-evaluation buffers
     mw::DevBuf<mw::EvalConst> consts;
@@ -441,3 +449,22 @@ struct mw_ctx {
     }
 };
 
+namespace mw {
+
+// Resampled star pass.  Along a star's sight the background's singularities
+// sit at Im x = +-2.17 theta in the magnitude variable, as along the
+// integral's sights, and the convolution window is a piece of that sight:
+// its background term folds onto a grid uniform in the magnitude offset at
+// spacing min(0.1 x 2.17 theta, 0.3 mag), theta the smallest over the
+// context's stars.
+// Tabulates mw_ctx::star_sin2_q from the star list (SoA, n stars).
+void tabulate_star_theta(mw_ctx& c, const double* soa, long n);
+// Background table index for the star pass under flattening q, or -1 when
+// the exact convolution is used (no stars, q outside the table, no level
+// fine enough, or no fewer nodes than the convolution's points).  Pure.
+int star_bg_level(const mw_ctx& c, double q);
+// That table (built on first use): rq (r / r_star, weight / r_star^3) pairs
+// and their number; nullptr and 0 for level -1.
+int star_bg_table(mw_ctx& c, int level, const double** rq, int* count);
+
+}  // namespace mw

@@ -110,6 +110,11 @@ struct StarLaunch {
     int n_streams;             // streams of the slot's model (batched kernel)
     const double* ratio;       // [n_conv] r_i / r_star of the star's context
     const double* kw;          // [n_conv] W_i coeff exp(-dx_i^2 / (2 sigma^2))
+    // bg_count > 0 (resampled mixed pass): the background term of each star's
+    // convolution from bg_count (r / r_star, weight / r_star^3) pairs of bg_rq
+    // instead of the n_conv points.
+    const double* bg_rq;
+    int bg_count;
 };
 
 // Constants of the stream separation pass (separation.c:37-254).

@@ -748,12 +748,19 @@ __global__ void reset_counters(unsigned* counters, int n)
 // bg is the background density times reff(r) 3 sigma / r^3 (or -1 when
 // q == 0); st[s] is the stream density without that factor and without the
 // stream's on/off flag.
+//
+// bg_count > 0 (resampled mixed pass, Hernquist profile): the background
+// term from the bg_count (r / r_star, weight / r_star^3) pairs of bg_tab, a
+// fold of the convolution onto a grid uniform in the magnitude offset; the
+// n points then carry the streams alone.
 template <int S, int BG, bool MIXED>
 __device__ __forceinline__ void star_densities(const double* __restrict__ star, long N, long i, int n,
                                                const EvalConst& K,
                                                const double* __restrict__ conv_ratio,
                                                const double* __restrict__ conv_kw, double& reff,
-                                               double& bg, double (&st)[kMaxStreams])
+                                               double& bg, double (&st)[kMaxStreams],
+                                               const double* __restrict__ bg_tab = nullptr,
+                                               int bg_count = 0)
 {
     const Sight<S> sight = make_sight<S, BG, MIXED>(star[i], star[N + i], star[2 * N + i], K);
     const double rstar = star[3 * N + i];
@@ -766,10 +773,29 @@ __device__ __forceinline__ void star_densities(const double* __restrict__ star, 
     // constant address space, where uniform loads become scalar loads.
     typedef const __attribute__((address_space(4))) double* ConstPtr;
     const ConstPtr ratio = (ConstPtr)conv_ratio, kw = (ConstPtr)conv_kw;
-    for (int j = 0; j < n; j++) {
-        const double r = rstar * ratio[j];
-        const double qw = kw[j] * (r * r * r);
-        point_terms<S, BG, MIXED>(r, qw, sight, K, bg, st);
+    bool folded = false;
+    if constexpr (MIXED && BG == kBgHernquist) {
+        if (bg_count > 0) {
+            folded = true;
+            double unused = 0.0;
+            for (int j = 0; j < n; j++) {
+                const double r = rstar * ratio[j];
+                const double qw = kw[j] * (r * r * r);
+                point_terms<S, kBgZeroQ, MIXED>(r, qw, sight, K, unused, st);
+            }
+            const ConstPtr bq = (ConstPtr)bg_tab;
+            double b = 0.0;
+#pragma unroll 2
+            for (int m = 0; m < bg_count; m++) bg_point<BG>(rstar * bq[2 * m], bq[2 * m + 1], sight.bg, K, b);
+            bg = b * (rstar * rstar * rstar);
+        }
+    }
+    if (!folded) {
+        for (int j = 0; j < n; j++) {
+            const double r = rstar * ratio[j];
+            const double qw = kw[j] * (r * r * r);
+            point_terms<S, BG, MIXED>(r, qw, sight, K, bg, st);
+        }
     }
     if constexpr (BG == kBgZeroQ) {
         bg = -1.0;
@@ -803,7 +829,8 @@ __device__ __forceinline__ void star_block(const StarLaunch& it, long lb,
 
     double reff, bg;
     double st[kMaxStreams];
-    star_densities<S, BG, MIXED>(it.star, N, i, it.n_conv, K, conv_ratio, conv_kw, reff, bg, st);
+    star_densities<S, BG, MIXED>(it.star, N, i, it.n_conv, K, conv_ratio, conv_kw, reff, bg, st, it.bg_rq,
+                                 it.bg_count);
 
     // Mixture (evaluation_optimized.c:674-716).
     DD out[kStarComps];
