@@ -72,6 +72,7 @@ private:
     struct Group {
         int bg;
         int first_item, n_items, first_block, n_blocks;
+        bool split = true;   // every item in the split layout (integral groups)
     };
     int device_ = 0, flags_ = 0, n_cu_ = 256;
     hipStream_t stream_ = nullptr;

@@ -734,6 +734,37 @@ __global__ __launch_bounds__(kBlock, WAVES) void integral_persistent(
     }
 }
 
+// Persistent form for a launch whose items are all in the split layout
+// (resampled, mixed, bg_stride 0): only integral_block_split is compiled in,
+// so the register allocation is that path's alone (121 VGPRs without spills
+// at the compiler's occupancy, 96 at 5 waves per SIMD; the general kernel
+// holds every path and needs 128 with spills).
+template <int BG, int WAVES>
+__global__ __launch_bounds__(kBlock, WAVES) void integral_persistent_split(
+    const RegionLaunch* __restrict__ items, int n_items, int total_blocks,
+    const EvalConst* __restrict__ consts, double* __restrict__ partials, unsigned* counter)
+{
+    __shared__ DD red[kBlock / 64][kIntComps];
+    __shared__ int next;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) next = (int)atomicAdd(counter, 1u);
+        __syncthreads();
+        const int b = next;
+        if (b >= total_blocks) break;
+        const RegionLaunch& it = items[find_item(items, n_items, b)];
+        double* prow = partials + (long)b * kIntComps * 2;
+        const long lb = b - it.block_begin;
+        switch (it.n_streams) {
+        case 0: integral_block_split<0, BG>(it, lb, consts, prow, red); break;
+        case 1: integral_block_split<1, BG>(it, lb, consts, prow, red); break;
+        case 2: integral_block_split<2, BG>(it, lb, consts, prow, red); break;
+        case 3: integral_block_split<3, BG>(it, lb, consts, prow, red); break;
+        default: integral_block_split<4, BG>(it, lb, consts, prow, red); break;
+        }
+    }
+}
+
 __global__ void reset_counters(unsigned* counters, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1241,13 +1272,23 @@ static PersistentFn pick_persistent(int bg, bool mixed, int v)
 
 int persistent_variants() { return kNumVariants; }
 
+// Split-layout launches: the variants' occupancy targets map onto the split
+// kernel's two (5 waves, or the compiler's); the unroll is the split path's own.
+static PersistentFn pick_persistent_split(int v)
+{
+    if (v >= 0 && v < kNumVariants && kVariantWaves[v] == 5) return integral_persistent_split<kBgHernquist, 5>;
+    return integral_persistent_split<kBgHernquist, 1>;
+}
+
 hipError_t launch_persistent(int bg, bool mixed, int variant, int grid, const RegionLaunch* d_items,
                              int n_items, int total_blocks, const EvalConst* d_consts,
-                             double* d_partials, unsigned* d_counter, hipStream_t stream)
+                             double* d_partials, unsigned* d_counter, hipStream_t stream, bool split)
 {
     if (total_blocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(pick_persistent(bg, mixed, variant), dim3(grid), dim3(kBlock), 0, stream,
-                       d_items, n_items, total_blocks, d_consts, d_partials, d_counter);
+    const PersistentFn fn = (split && mixed && bg == kBgHernquist) ? pick_persistent_split(variant)
+                                                                    : pick_persistent(bg, mixed, variant);
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(kBlock), 0, stream, d_items, n_items, total_blocks, d_consts,
+                       d_partials, d_counter);
     return hipGetLastError();
 }
 
