@@ -537,14 +537,20 @@ __device__ __forceinline__ void integral_block_split(const RegionLaunch& it, lon
             const double p = sight.st[s].p, mm = sight.st[s].m;
             const double d = p * h;
             const double c = exp2_neg(2.0 * d * d);
+            // t rises along the sight, so a row's nodes all lie beyond the
+            // negligible bound T = sqrt(min t^2 + 70) exactly when t0 > T or
+            // t1 < -T; t0 advances by n d per row (a rounding per row, far
+            // inside the bound's margin).
             const double floor_t2 =
                 min_t2(fma(p, lo, -mm), fma(p, fma((double)R * n - 1.0, h, lo), -mm)) + kNegligibleLog2;
+            const double T = floor_t2 * rsqrt_nr(floor_t2);
+            const double dn = d * n, d1 = d * (n - 1);
             const ConstPtr wq = (ConstPtr)(it.st_rq[s]);
             double G = 0.0, Rr = 0.0, a = 0.0;
-            for (int k = 0; k < R; k++) {
-                const double t0 = fma(p, fma((double)k * n, h, lo), -mm);
-                const double t1 = fma(d, (double)(n - 1), t0);
-                if (__all(min_t2(t0, t1) > floor_t2)) {
+            double t0 = fma(p, lo, -mm);
+            for (int k = 0; k < R; k++, t0 += dn) {
+                const double t1 = t0 + d1;
+                if (__all(t0 > T || t1 < -T)) {
                     G = 0.0;   // negligible here; a later row before the peak restarts
                     continue;
                 }
