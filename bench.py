@@ -287,7 +287,7 @@ def bench_batch(args):
         choice = {"variant": args.variant[0], "blocks_per_cu": args.variant[1], "table": []}
     else:
         choice = autotune.tune(groups, vecs, head_prec, proxy_vectors=1, reps=1,
-                               blocks_per_cu=(2, 4), use_cache=False, log=log)
+                               blocks_per_cu=(2, 4, 5), use_cache=False, log=log)
     tune_s = time.perf_counter() - tune_t0
     log(f"autotuner ({args.batch_precision}): variant {choice['variant']} blocks/CU "
         f"{choice['blocks_per_cu']} ({tune_s:.1f} s)")
@@ -302,7 +302,7 @@ def bench_batch(args):
     second = None
     if not args.no_secondary:
         tune2 = autotune.tune(groups, vecs, other_prec, proxy_vectors=1, reps=1,
-                              blocks_per_cu=(2, 4), use_cache=False, log=log)
+                              blocks_per_cu=(2, 4, 5), use_cache=False, log=log)
         log("batched mixed precision, the other quadrature")
         v2, lat2, _, tm2 = timed_batch(pairs, vecs, other_prec, tune2, max(1, min(args.steps, 2)), 1)
         second = (v2, lat2, tm2, tune2)

@@ -3,7 +3,7 @@
 For a batch signature (each wedge's r x mu x nu grid, convolution points,
 stream count, star count and vector count, plus the precision) it times
 every kernel variant (inner-loop unroll 1/2/4 x occupancy target default/5
-waves per EU) at several persistent grid sizes (1, 2, 4 or 8 workgroups per
+waves per EU) at several persistent grid sizes (1, 2, 4, 5 or 8 workgroups per
 compute unit), picks the fastest and caches the choice in a JSON file.
 
 Timing runs on a proxy of the batch: the same resident wedges with at most
@@ -23,7 +23,7 @@ from . import separation as sep
 
 DEFAULT_CACHE = Path(os.environ.get("MW_TUNING_CACHE",
                                     Path.home() / ".cache" / "mw_separation" / "tuning.json"))
-BLOCKS_PER_CU = (1, 2, 4, 8)
+BLOCKS_PER_CU = (1, 2, 4, 5, 8)
 # Bumped whenever the library's table of kernel variants changes meaning
 # (variant v = unroll {2, 4} x occupancy target {compiler's, 4, 5} waves),
 # so that cached choices of an older table are not reused.

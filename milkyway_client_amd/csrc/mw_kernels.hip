@@ -567,7 +567,7 @@ __device__ __forceinline__ void integral_block_split(const RegionLaunch& it, lon
                     Rr *= c;
                 }
             }
-            acc[1 + s] = a;
+            acc[1 + s] = a * exp2_neg(sight.st[s].e);   // the sight's minimum, taken out of every term
         }
     }
     double out[1 + S];
@@ -575,10 +575,9 @@ __device__ __forceinline__ void integral_block_split(const RegionLaunch& it, lon
     for (int c = 0; c <= S; c++) out[c] = 0.0;
     if (valid) {
         const double wnu = it.col[3 * C + cc];
-        const Sight<S> again = make_sight<S, BG, true>(it.col[cc], it.col[C + cc], it.col[2 * C + cc], K);
         out[0] = acc[0] * wnu;
 #pragma unroll
-        for (int s = 0; s < S; s++) out[1 + s] = acc[1 + s] * wnu * K.on[s] * exp2_neg(again.st[s].e);
+        for (int s = 0; s < S; s++) out[1 + s] = acc[1 + s] * wnu * K.on[s];
     }
     write_block_f64<S>(out, prow, red);
 }
