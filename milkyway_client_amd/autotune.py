@@ -2,7 +2,7 @@
 
 For a batch signature (each wedge's r x mu x nu grid, convolution points,
 stream count, star count and vector count, plus the precision) it times
-every kernel variant (inner-loop unroll 1/2/4 x occupancy target default/5
+every kernel variant (inner-loop unroll 2/4 x occupancy target default/4/5
 waves per EU) at several persistent grid sizes (1, 2, 4, 5 or 8 workgroups per
 compute unit), picks the fastest and caches the choice in a JSON file.
 
