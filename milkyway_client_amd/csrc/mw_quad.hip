@@ -147,10 +147,11 @@ int bg_grid_level(const RegionDev& rd, double q)
     // the singularities' distance from the real axis (2.17 theta), and at
     // most 0.3 mag, where the profile's r^-4 tail (an exponential of rate
     // 1.84 per mag in x) sets the interpolation error instead.  Measured on
-    // the fold of a dense quadrature over 1-105 kpc, every (rg_min, P, r0)
-    // from (0.5-8.4 kpc, 0.5-5, 1-14 kpc), towards and away from the
-    // centre: relative error of the integral below 1.1e-10 (4.6e-11 at
-    // 0.15 of the distance, 5.2e-12 at 0.15 and 0.25 mag, 5e-10 at 0.35 mag).
+    // the fold of the full-size point set (700 r-steps x 120), every
+    // (rg_min, P, r0) from (0.5-8.4 kpc, 0.5-5, 1-14 kpc), towards and away
+    // from the centre, each sight at its own theta: relative error of the
+    // integral below 6.5e-10, the worst at the cap (4.7e-11 at 0.15 of the
+    // distance).
     const double dg = std::min(0.2 * kMagPerLn * bg_theta(rd, q), kDeltaBgMag);
     if (!(dg > 0.0)) return -1;
     const int t = level_for(dg, kDeltaBgMag);

@@ -1,0 +1,131 @@
+"""The spacing rules of the split layout and of the resampled star pass,
+pinned on the library's own fold (mw_resample_weights), on the CPU.
+
+Along a line of sight the background 1 / (rg (rg + r0)^3) has its
+singularities at r = r_c +- i rg_min / sqrt(P), which in the magnitude
+variable x = 2.17 ln r sit at Im x = +-2.17 theta with sin theta =
+rg_min / 8.5 (pi / 2 for a sight pointing away from the centre).  The
+library folds
+  * the integral's background onto a magnitude grid at
+    min(0.2 x 2.17 theta, 0.3 mag), rounded down to 0.3 x 2^(-t/2) mag;
+  * each star's background over its convolution offsets at
+    min(0.1 x 2.17 theta, 0.3 mag), on the same levels.
+These tests evaluate exact and folded sums for sights at the rules' worst
+cases (every sight takes its own theta, with none of the margin the
+library's tables add) and hold them to the bounds the library's comments
+state.  A star fold at the integral's 0.2 must fail its bar: the short
+convolution window sees the interpolation error without the cancellation
+a whole sight gives.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from milkyway_client_amd import separation as sep
+from oracle import oracle as O
+
+K_MAG = 5.0 / math.log(10.0)
+D_BG_MAX = 0.3
+
+
+def level_delta(dg):
+    """The coarsest level 0.3 x 2^(-t/2) no coarser than dg."""
+    t = max(0, math.ceil(-2.0 * math.log2(min(dg, D_BG_MAX) / D_BG_MAX) - 1e-12))
+    return D_BG_MAX * 2.0 ** (-0.5 * t)
+
+
+def sight(rg_min, P, away):
+    """(r_c, theta) of a sight with closest approach rg_min in the q-scaled
+    metric and P = u_x^2 + u_y^2 + u_z^2 / q^2 (r_c^2 P + rg_min^2 = 8.5^2)."""
+    rc = math.sqrt(max(8.5 ** 2 - rg_min ** 2, 0.0) / P)
+    if away:
+        return -rc, 0.5 * math.pi
+    return rc, math.asin(rg_min / 8.5)
+
+
+def background(r, rg_min, P, rc, r0):
+    rg = np.sqrt(P * (r - rc) ** 2 + rg_min ** 2)
+    return 1.0 / (rg * (rg + r0) ** 3)
+
+
+@pytest.fixture(scope="module")
+def points():
+    """The full-size point set (700 r-steps x 120 convolution points) in
+    magnitude, with the integral's weights."""
+    R, n = 700, 120
+    t = O.r_constants(n, R, 16.0, 6.5 / R, 1.0, 1, 0.0, 1.0)
+    rp = np.asarray(t["r_point"]).reshape(R, n)
+    w = (np.asarray(t["irv"]) * np.asarray(t["reff_xr_rp3"]))[:, None] * np.asarray(t["qw_r3_N"]).reshape(R, n)
+    g = 5 * (np.log10(rp * 1000) - 1) + 4.2
+    return rp.ravel(), g.ravel(), w.ravel()
+
+
+def test_background_grid_rule(points):
+    """Integral background on its own magnitude grid: worst relative error
+    over the rule's parameter range within 1e-9 (6.4e-10 measured, at the
+    0.3 mag cap with rg_min = 6 kpc; the library's tables add a margin)."""
+    rp, g, w = points
+    folds = {}
+    worst = 0.0
+    for rg_min in (0.5, 2.0, 6.0, 8.4):
+        for P in (0.5, 1.0, 5.0):
+            for away in (False, True):
+                rc, theta = sight(rg_min, P, away)
+                d = level_delta(0.2 * K_MAG * theta)
+                if d not in folds:
+                    folds[d] = sep.resample_weights(g, w, d)
+                lo, K = folds[d]
+                rm = 10 ** ((lo + d * np.arange(K.size) - 4.2) / 5 + 1) / 1000
+                for r0 in (1.0, 4.0, 14.0):
+                    exact = np.sum(w * background(rp, rg_min, P, rc, r0))
+                    got = np.sum(K * background(rm, rg_min, P, rc, r0))
+                    worst = max(worst, abs(got / exact - 1.0))
+    print(f"background grid: worst {worst:.2e} over {len(folds)} spacings")
+    assert worst <= 1e-9
+
+
+def conv_offsets(n):
+    """The star convolution's magnitude offsets and weights kw ratio^3
+    (the reference's Gauss-Legendre points over +-3 x 0.6 mag)."""
+    x, wg = np.polynomial.legendre.leggauss(n)
+    dx = 3 * 0.6 * x
+    kw = wg * np.exp(-dx * dx / (2 * 0.36)) / (0.6 * math.sqrt(2 * math.pi))
+    return dx, kw * 10 ** (3 * dx / 5)
+
+
+def star_fold_worst(c):
+    worst = 0.0
+    for n in (60, 120):
+        dx, w3 = conv_offsets(n)
+        for rg_min in (0.3, 2.0, 4.0, 8.0):
+            for P in (0.5, 1.0, 11.0):
+                for away in (False, True):
+                    rc, theta = sight(rg_min, P, away)
+                    d = level_delta(c * K_MAG * theta)
+                    lo, K = sep.resample_weights(dx, w3, d)
+                    if K.size >= n:
+                        continue                       # the library keeps the exact convolution
+                    ratio_m = 10 ** ((lo + d * np.arange(K.size)) / 5)
+                    for r0 in (1.0, 4.0, 14.0):
+                        for rstar in (1.0, 3.0, 8.0, 20.0, 60.0):
+                            exact = np.sum(w3 * background(rstar * 10 ** (dx / 5), rg_min, P, rc, r0))
+                            got = np.sum(K * background(rstar * ratio_m, rg_min, P, rc, r0))
+                            worst = max(worst, abs(got / exact - 1.0))
+    return worst
+
+
+def test_star_background_fold_rule():
+    """Star pass: a star's background term folded over its convolution
+    offsets, worst relative error within 5e-10 (2.5e-10 measured)."""
+    worst = star_fold_worst(0.1)
+    print(f"star fold at 0.1: worst {worst:.2e}")
+    assert worst <= 5e-10
+
+
+def test_star_fold_at_the_integrals_spacing_fails():
+    """The integral's 0.2 on a star's window: off by far more than the bar
+    allows (2.4e-7 measured), which is why the star pass takes 0.1."""
+    worst = star_fold_worst(0.2)
+    print(f"star fold at 0.2: worst {worst:.2e}")
+    assert worst > 1e-8
