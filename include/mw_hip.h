@@ -52,7 +52,9 @@ extern "C" {
  * about 1e-10 per sight, from up to hundreds of times fewer density
  * evaluations on large wedges.  Regions where it would not pay, and
  * background modes other than the Hernquist profile, use the exact tables.
- * The star pass is the exact one. */
+ * In the star pass the streams keep the exact convolution; each star's
+ * background term is folded over the convolution's magnitude offsets where
+ * that takes fewer nodes. */
 #define MW_PREC_RESAMPLED 2
 
 /* The weight folding behind MW_PREC_RESAMPLED, on the host, for any point
@@ -63,6 +65,12 @@ extern "C" {
  * only when capacity >= M), or -1 for bad input. */
 long mw_resample_weights(const double* g, const double* w, long n, double delta, int degree,
                          double* g_lo, double* K, long capacity);
+/* The same with the grid starting pad nodes below the lowest point and
+ * ending pad nodes above the highest (mw_resample_weights: pad = degree / 2
+ * + 1, every stencil centred; a smaller pad shifts the stencils at the ends
+ * inwards, as the library's background and star-pass tables do with 4). */
+long mw_resample_weights_padded(const double* g, const double* w, long n, double delta, int degree,
+                                int pad, double* g_lo, double* K, long capacity);
 
 /* ------------------------------------------------------------------------
  * Drop-in entry points

@@ -160,8 +160,11 @@ constexpr int kResampleDegree = 15;   // 16-node stencils
 constexpr int kResampleRow = 32;      // grid nodes per pseudo r-step row (padding < 32)
 
 // K over the grid g_lo + m delta, m < M; returns M (or -1 on bad input).
+// The grid starts pad nodes below the lowest point and ends pad nodes above
+// the highest; pad < 0: degree / 2 + 1, every stencil centred on its point's
+// interval (a smaller pad shifts the stencils at the ends inwards).
 long resample_quadrature(const double* g, const double* w, long n, double delta, int degree,
-                         double* g_lo, std::vector<double>* K);
+                         double* g_lo, std::vector<double>* K, int pad = -1);
 // The same fold onto a given grid lo + m delta, m < M (the caller leaves
 // room for the stencils below and above the points).
 long resample_quadrature_at(const double* g, const double* w, long n, double delta, int degree,
@@ -177,7 +180,7 @@ long resample_quadrature_at(const double* g, const double* w, long n, double del
 // q delta on every q-th node (0 elsewhere), node m at r = lo + m delta.
 // Returns M.
 long build_resampled_tables(const std::vector<double>& rq, const std::vector<RStep>& rstep, int n_conv,
-                            double delta, bool in_r, int bg_stride, RegionTables* out);
+                            double delta, bool in_r, int bg_stride, RegionTables* out, int pad = -1);
 
 // Star-pass constants shared by every star of a context: r_i / r_star and
 // W_i * coeff * exp(-dx_i^2 / (2 sigma^2)).

@@ -37,6 +37,12 @@ constexpr double kMagPerLn = 2.1714724095162588;   // 5 / ln 10
 constexpr double kDeltaMag = 0.02;                  // coarsest magnitude spacing
 constexpr double kDeltaKpc = 2.0;                   // coarsest distance spacing
 constexpr double kDeltaBgMag = 0.3;                 // coarsest background-grid spacing
+// Background and star-pass tables: the fold's lattice ends 4 nodes beyond
+// the points (not 8, every stencil centred): the end stencils shift inwards
+// over points that carry the convolution's tail weights, and the measured
+// worst cases of both rules are unchanged (6.4e-10, 2.5e-10) with 8 fewer
+// nodes.
+constexpr int kBgPad = 4;
 
 // Spacing of table index t (see kQuadLevels).
 double table_delta(int t)
@@ -247,9 +253,8 @@ long table_nodes(const RegionDev& rd, int level, int stride)
 // padding).
 long bg_table_nodes(const RegionDev& rd, int bg_level)
 {
-    const int half = kResampleDegree / 2;
     const double d = kDeltaBgMag * std::pow(2.0, -0.5 * bg_level);
-    return (long)std::ceil(rd.g_span / d) + 2 * half + 3;
+    return std::max((long)std::ceil(rd.g_span / d) + 2 * kBgPad + 1, (long)kResampleDegree + 1);
 }
 
 }  // namespace
@@ -370,7 +375,8 @@ int region_tables(RegionDev& rd, const GridPlan& g, TableRef* out)
         if (!bslot) {
             RegionTables t;
             const long M = build_resampled_tables(rd.h_rq, rd.h_rstep, rd.n_conv,
-                                                  kDeltaBgMag * std::pow(2.0, -0.5 * g.bg_level), false, 0, &t);
+                                                  kDeltaBgMag * std::pow(2.0, -0.5 * g.bg_level), false, 0, &t,
+                                                  kBgPad);
             if (M < 1) return fail(MW_E_INVALID, "resampled quadrature failed");
             int rc;
             if ((rc = upload_table(t, M, true, &bslot))) return rc;
@@ -408,9 +414,9 @@ namespace {
 // the convolution offsets).
 long star_bg_nodes(const mw_ctx& c, double d)
 {
-    const int half = kResampleDegree / 2;
-    const double span = c.conv_dx.back() - c.conv_dx.front();
-    return (long)std::ceil(span / d) + 2 * half + 3;
+    const auto mm = std::minmax_element(c.conv_dx.begin(), c.conv_dx.end());
+    const double span = *mm.second - *mm.first;
+    return std::max((long)std::ceil(span / d) + 2 * kBgPad + 1, (long)kResampleDegree + 1);
 }
 
 double star_bg_delta(int level) { return kDeltaBgMag * std::pow(2.0, -0.5 * level); }
@@ -455,7 +461,7 @@ int star_bg_table(mw_ctx& c, int level, const double** rq, int* count)
         double lo;
         const double d = star_bg_delta(level);
         const long M = resample_quadrature(c.conv_dx.data(), c.conv_w3.data(), (long)c.conv_dx.size(), d,
-                                           kResampleDegree, &lo, &K);
+                                           kResampleDegree, &lo, &K, kBgPad);
         if (M < 1) return fail(MW_E_INVALID, "star quadrature failed");
         std::vector<double> pairs(2 * M);
         for (long m = 0; m < M; m++) {

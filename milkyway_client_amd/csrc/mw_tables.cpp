@@ -551,7 +551,7 @@ long resample_quadrature_at(const double* g, const double* w, long n, double del
 }
 
 long resample_quadrature(const double* g, const double* w, long n, double delta, int degree,
-                         double* g_lo, std::vector<double>* K)
+                         double* g_lo, std::vector<double>* K, int pad)
 {
     if (n < 1 || !(delta > 0.0) || degree < 1 || degree > 31) return -1;
     double gmin = g[0], gmax = g[0];
@@ -559,15 +559,15 @@ long resample_quadrature(const double* g, const double* w, long n, double delta,
         gmin = std::min(gmin, g[i]);
         gmax = std::max(gmax, g[i]);
     }
-    const int half = degree / 2;
-    const double lo = gmin - (half + 1) * delta;
-    const long M = (long)std::ceil((gmax - lo) / delta) + half + 2;
+    if (pad < 0) pad = degree / 2 + 1;
+    const double lo = gmin - pad * delta;
+    const long M = std::max((long)std::ceil((gmax - lo) / delta) + pad + 1, (long)degree + 1);
     *g_lo = lo;
     return resample_quadrature_at(g, w, n, delta, degree, lo, M, K);
 }
 
 long build_resampled_tables(const std::vector<double>& rq, const std::vector<RStep>& rstep, int n_conv,
-                            double delta, bool in_r, int bg_stride, RegionTables* out)
+                            double delta, bool in_r, int bg_stride, RegionTables* out, int pad)
 {
     const long R = (long)rstep.size();
     const long N = R * n_conv;
@@ -589,7 +589,7 @@ long build_resampled_tables(const std::vector<double>& rq, const std::vector<RSt
     long M;
     std::vector<double> K, Kb;
     if (bg_stride == 0) {
-        M = resample_quadrature(g.data(), w.data(), N, delta, kResampleDegree, &lo, &K);
+        M = resample_quadrature(g.data(), w.data(), N, delta, kResampleDegree, &lo, &K, pad);
         if (M < 1) return -1;
     } else {
         // Two folds on one lattice: the streams at spacing delta, the
@@ -721,13 +721,19 @@ bool make_eval_const(const ASTRONOMY_PARAMETERS& ap, EvalConst* k, std::string* 
 
 }  // namespace mw
 
-extern "C" long mw_resample_weights(const double* g, const double* w, long n, double delta, int degree,
-                                    double* g_lo, double* K, long capacity)
+extern "C" long mw_resample_weights_padded(const double* g, const double* w, long n, double delta,
+                                           int degree, int pad, double* g_lo, double* K, long capacity)
 {
-    if (!g || !w || !g_lo) return -1;
+    if (!g || !w || !g_lo || pad < 0) return -1;
     std::vector<double> k;
-    const long M = mw::resample_quadrature(g, w, n, delta, degree, g_lo, &k);
+    const long M = mw::resample_quadrature(g, w, n, delta, degree, g_lo, &k, pad);
     if (M < 1) return -1;
     if (K && capacity >= M) std::memcpy(K, k.data(), sizeof(double) * M);
     return M;
+}
+
+extern "C" long mw_resample_weights(const double* g, const double* w, long n, double delta, int degree,
+                                    double* g_lo, double* K, long capacity)
+{
+    return mw_resample_weights_padded(g, w, n, delta, degree, degree / 2 + 1, g_lo, K, capacity);
 }

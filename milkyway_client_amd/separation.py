@@ -52,7 +52,7 @@ EXPORTS = [
     "mw_get_step", "mw_get_min_parameters", "mw_get_max_parameters", "mw_hessian",
     "mw_ctx_hessian", "mw_matrix_invert", "mw_errors",
     "mw_write_checkpoint", "mw_read_checkpoint", "mw_calculate_progress",
-    "mw_resample_weights",
+    "mw_resample_weights", "mw_resample_weights_padded",
 ]
 
 # int (*)(void* user, const double* vectors, int count, double* values)
@@ -238,6 +238,9 @@ def load_library() -> C.CDLL:
     L.mw_calculate_progress.restype = C.c_double
     L.mw_resample_weights.argtypes = [_dp, _dp, C.c_long, C.c_double, C.c_int, _dp, _dp, C.c_long]
     L.mw_resample_weights.restype = C.c_long
+    L.mw_resample_weights_padded.argtypes = [_dp, _dp, C.c_long, C.c_double, C.c_int, C.c_int, _dp, _dp,
+                                             C.c_long]
+    L.mw_resample_weights_padded.restype = C.c_long
     _lib = L
     return L
 
@@ -546,17 +549,20 @@ def errors(hessian, number_stars: int):
     return sigma, sd
 
 
-def resample_weights(g, w, delta: float, degree: int = 15):
-    """mw_resample_weights: grid start, grid weights K."""
+def resample_weights(g, w, delta: float, degree: int = 15, pad: int = None):
+    """mw_resample_weights (pad None: every stencil centred) or
+    mw_resample_weights_padded: grid start, grid weights K."""
     g = np.ascontiguousarray(g, dtype=np.float64)
     w = np.ascontiguousarray(w, dtype=np.float64)
     lo = C.c_double()
     L = load_library()
-    M = L.mw_resample_weights(g.ctypes.data_as(_dp), w.ctypes.data_as(_dp), g.size, delta, degree,
-                              C.byref(lo), None, 0)
+    if pad is None:
+        pad = degree // 2 + 1
+    M = L.mw_resample_weights_padded(g.ctypes.data_as(_dp), w.ctypes.data_as(_dp), g.size, delta, degree,
+                                     pad, C.byref(lo), None, 0)
     if M < 1:
         raise ValueError("bad resampling input")
     K = np.zeros(M)
-    L.mw_resample_weights(g.ctypes.data_as(_dp), w.ctypes.data_as(_dp), g.size, delta, degree,
-                          C.byref(lo), K.ctypes.data_as(_dp), M)
+    L.mw_resample_weights_padded(g.ctypes.data_as(_dp), w.ctypes.data_as(_dp), g.size, delta, degree,
+                                 pad, C.byref(lo), K.ctypes.data_as(_dp), M)
     return lo.value, K

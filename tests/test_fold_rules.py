@@ -10,8 +10,9 @@ library folds
     min(0.2 x 2.17 theta, 0.3 mag), rounded down to 0.3 x 2^(-t/2) mag;
   * each star's background over its convolution offsets at
     min(0.1 x 2.17 theta, 0.3 mag), on the same levels.
-These tests evaluate exact and folded sums for sights at the rules' worst
-cases (every sight takes its own theta, with none of the margin the
+Both folds end 4 nodes beyond their points (PAD), as the library's tables
+do.  These tests evaluate exact and folded sums for sights at the rules'
+worst cases (every sight takes its own theta, with none of the margin the
 library's tables add) and hold them to the bounds the library's comments
 state.  A star fold at the integral's 0.2 must fail its bar: the short
 convolution window sees the interpolation error without the cancellation
@@ -27,6 +28,7 @@ from oracle import oracle as O
 
 K_MAG = 5.0 / math.log(10.0)
 D_BG_MAX = 0.3
+PAD = 4
 
 
 def level_delta(dg):
@@ -74,7 +76,7 @@ def test_background_grid_rule(points):
                 rc, theta = sight(rg_min, P, away)
                 d = level_delta(0.2 * K_MAG * theta)
                 if d not in folds:
-                    folds[d] = sep.resample_weights(g, w, d)
+                    folds[d] = sep.resample_weights(g, w, d, pad=PAD)
                 lo, K = folds[d]
                 rm = 10 ** ((lo + d * np.arange(K.size) - 4.2) / 5 + 1) / 1000
                 for r0 in (1.0, 4.0, 14.0):
@@ -103,7 +105,7 @@ def star_fold_worst(c):
                 for away in (False, True):
                     rc, theta = sight(rg_min, P, away)
                     d = level_delta(c * K_MAG * theta)
-                    lo, K = sep.resample_weights(dx, w3, d)
+                    lo, K = sep.resample_weights(dx, w3, d, pad=PAD)
                     if K.size >= n:
                         continue                       # the library keeps the exact convolution
                     ratio_m = 10 ** ((lo + d * np.arange(K.size)) / 5)
@@ -125,7 +127,7 @@ def test_star_background_fold_rule():
 
 def test_star_fold_at_the_integrals_spacing_fails():
     """The integral's 0.2 on a star's window: off by far more than the bar
-    allows (2.4e-7 measured), which is why the star pass takes 0.1."""
+    allows (6.1e-7 measured), which is why the star pass takes 0.1."""
     worst = star_fold_worst(0.2)
     print(f"star fold at 0.2: worst {worst:.2e}")
     assert worst > 1e-8
