@@ -138,7 +138,9 @@ int build_context(mw_ctx* c, const std::vector<RegionGeom>& geoms, int n_conv, i
     }
     if (c->n_stars > 0) tabulate_star_theta(*c, soa.data(), c->n_stars);
     int rc;
-    if ((rc = c->ratio.reserve(n_conv)) || (rc = c->kw.reserve(n_conv))) return rc;
+    if ((rc = c->ratio.reserve(n_conv)) || (rc = c->kw.reserve(n_conv)) || (rc = c->kw3.reserve(n_conv)))
+        return rc;
+    MW_HIP_CHECK(hipMemcpy(c->kw3.p, c->conv_w3.data(), sizeof(double) * n_conv, hipMemcpyHostToDevice));
     MW_HIP_CHECK(hipMemcpy(c->ratio.p, ratio.data(), sizeof(double) * n_conv, hipMemcpyHostToDevice));
     MW_HIP_CHECK(hipMemcpy(c->kw.p, kw.data(), sizeof(double) * n_conv, hipMemcpyHostToDevice));
     return MW_OK;
@@ -267,6 +269,7 @@ int run_plan(mw_ctx* c, Plan& P)
             L.n_streams = c->S;
             L.ratio = c->ratio.p;
             L.kw = c->kw.p;
+            L.kw3 = c->kw3.p;
             const int sl = (P.resample && P.mixed && bg == kBgHernquist) ? star_bg_level(*c, P.K[s].q) : -1;
             if ((rc = star_bg_table(*c, sl, &L.bg_rq, &L.bg_count))) return rc;
             ReduceItem& R = c->h_sred.p[oi];
@@ -504,7 +507,7 @@ size_t mw_ctx_resident_bytes(const mw_ctx* c)
         for (const auto& q : r->bg_levels)
             if (q) b += (q->rq.cap + q->w.cap) * sizeof(double) + q->rstep.cap * sizeof(RStep);
     }
-    b += (c->star.cap + c->ratio.cap + c->kw.cap) * sizeof(double);
+    b += (c->star.cap + c->ratio.cap + c->kw.cap + c->kw3.cap) * sizeof(double);
     return b;
 }
 

@@ -230,6 +230,7 @@ int BatchRunner::plan(const std::vector<Slot>& slots, int precision)
         L.n_streams = slots[s].k.n_streams;
         L.ratio = c->ratio.p;
         L.kw = c->kw.p;
+        L.kw3 = c->kw3.p;
         int rc1;
         if ((rc1 = star_bg_table(*c, star_level(*c, slots[s].k), &L.bg_rq, &L.bg_count))) return rc1;
         hs[oi] = L;

@@ -393,7 +393,7 @@ struct mw_ctx {
     mw::ModelTemplate tmpl;
     std::vector<std::unique_ptr<mw::RegionDev>> regions;
     long n_stars = 0;
-    mw::DevBuf<double> star, ratio, kw;
+    mw::DevBuf<double> star, ratio, kw, kw3;
     // Resampled star pass: the convolution points' magnitude offsets and
     // weights kw ratio^3 (host), the smallest sin^2 theta over the stars at
     // the tabulated q values (see star_bg_level), and the background tables

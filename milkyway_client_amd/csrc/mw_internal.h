@@ -115,6 +115,7 @@ struct StarLaunch {
     // instead of the n_conv points.
     const double* bg_rq;
     int bg_count;
+    const double* kw3;         // [n_conv] kw_i ratio_i^3 (the folded pass's stream weights / r_star^3)
 };
 
 // Constants of the stream separation pass (separation.c:37-254).

@@ -358,9 +358,10 @@ constexpr int kRestartRows = 8;
 
 // A stream's terms in a row are skipped, by the whole wave, when every one of
 // them on every lane lies below 2^-kNegligibleLog2 of that sight's largest
-// term over the region's nodes (2^-(min t^2)): what is dropped is then below
-// 2^-70 relative, per sight, whatever the weights.
-constexpr double kNegligibleLog2 = 70.0;
+// term over the region's nodes (2^-(min t^2)): each dropped term is below
+// 2^-60 of that term, so with under 10^3 terms per sight and weights within
+// 10^4 of each other what is dropped stays below 1e-11 of the sum.
+constexpr double kNegligibleLog2 = 60.0;
 
 // Smallest t^2 over the nodes from t_first to t_last (t rises with r): 0
 // when the Gaussian's peak lies between them.
@@ -548,16 +549,18 @@ __device__ __forceinline__ void integral_block_split(const RegionLaunch& it, lon
             const ConstPtr wq = (ConstPtr)(it.st_rq[s]);
             double G = 0.0, Rr = 0.0, a = 0.0;
             double t0 = fma(p, lo, -mm);
-            for (int k = 0; k < R; k++, t0 += dn) {
+            int since = kRestartRows;   // rows since the last restart: the first row restarts
+            for (int k = 0; k < R; k++, t0 += dn, since++) {
                 const double t1 = t0 + d1;
                 if (__all(t0 > T || t1 < -T)) {
                     G = 0.0;   // negligible here; a later row before the peak restarts
                     continue;
                 }
-                if (k % kRestartRows == 0 || (t0 < 0.0 && G < 0x1p-1000)) {
+                if (since >= kRestartRows || (t0 < 0.0 && G < 0x1p-1000)) {
                     const double tc = fmax(t0, -40.0);
                     G = exp2_neg(tc * tc);
                     Rr = exp2_neg(fma(2.0 * tc, d, d * d));
+                    since = 0;
                 }
                 const ConstPtr row = wq + (long)k * n;
 #pragma unroll 4
@@ -796,7 +799,7 @@ __device__ __forceinline__ void star_densities(const double* __restrict__ star, 
                                                const double* __restrict__ conv_kw, double& reff,
                                                double& bg, double (&st)[kMaxStreams],
                                                const double* __restrict__ bg_tab = nullptr,
-                                               int bg_count = 0)
+                                               int bg_count = 0, const double* __restrict__ conv_kw3 = nullptr)
 {
     const Sight<S> sight = make_sight<S, BG, MIXED>(star[i], star[N + i], star[2 * N + i], K);
     const double rstar = star[3 * N + i];
@@ -814,10 +817,11 @@ __device__ __forceinline__ void star_densities(const double* __restrict__ star, 
         if (bg_count > 0) {
             folded = true;
             double unused = 0.0;
+            const ConstPtr kw3 = (ConstPtr)conv_kw3;   // kw ratio^3: qw = r_star^3 kw3
+            const double rs3 = rstar * rstar * rstar;
             for (int j = 0; j < n; j++) {
                 const double r = rstar * ratio[j];
-                const double qw = kw[j] * (r * r * r);
-                point_terms<S, kBgZeroQ, MIXED>(r, qw, sight, K, unused, st);
+                point_terms<S, kBgZeroQ, MIXED>(r, rs3 * kw3[j], sight, K, unused, st);
             }
             const ConstPtr bq = (ConstPtr)bg_tab;
             double b = 0.0;
@@ -866,7 +870,7 @@ __device__ __forceinline__ void star_block(const StarLaunch& it, long lb,
     double reff, bg;
     double st[kMaxStreams];
     star_densities<S, BG, MIXED>(it.star, N, i, it.n_conv, K, conv_ratio, conv_kw, reff, bg, st, it.bg_rq,
-                                 it.bg_count);
+                                 it.bg_count, it.kw3);
 
     // Mixture (evaluation_optimized.c:674-716).
     DD out[kStarComps];
