@@ -417,6 +417,45 @@ def test_batch_equals_single(wedges, graph):
             assert all(same(g, r) for g, r in zip(got, singles))
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_mixing_split_and_strided_layouts(graph, monkeypatch):
+    """One batch over a wedge in the split layout and a wedge whose
+    background stays on the distance grid (built with MW_HIP_BG_GRID=0):
+    the launch then runs the general persistent kernel instead of the
+    split-only one, and every result equals its single evaluation bitwise;
+    the split wedge alone runs the split-only kernel, with the same results."""
+    p1, s1 = synth.workload(171, LONG_R, 3)
+    p2, s2 = synth.workload(172, LONG_R, 2)
+    for st in p2.streams:
+        st.parameters[4] = 1.0
+    c1 = sep.Context(ParamsHolder(p1), StarsHolder(s1), timing=True)
+    monkeypatch.setenv("MW_HIP_BG_GRID", "0")
+    c2 = sep.Context(ParamsHolder(p2), StarsHolder(s2), timing=True)
+    monkeypatch.delenv("MW_HIP_BG_GRID")
+    try:
+        v1 = synth.parameter_vectors(171, p1, 3)
+        v2 = synth.parameter_vectors(172, p2, 2)
+        v2[:, 2 + 5::6] = 1.0                      # every stream 1 kpc wide: a fine grid
+        single1 = [c1.eval(v, precision=sep.PREC_RESAMPLED)[0] for v in v1]
+        t1 = c1.timing()
+        single2 = [c2.eval(v, precision=sep.PREC_RESAMPLED)[0] for v in v2]
+        t2 = c2.timing()
+        # the first wedge's sights in the split layout (background on its own
+        # grid), the second's with the background on the stream grid
+        assert t1.integral_points_background < 0.5 * t1.integral_points
+        assert t2.integral_points_background > 0.2 * t2.integral_points
+        with sep.Batch([(c1, 3), (c2, 2)], precision=sep.PREC_RESAMPLED, graph=graph) as b:
+            for _ in range(2 if graph else 1):
+                got = b.run([v1, v2])
+                assert all(same(g, r) for g, r in zip(got, single1 + single2))
+        with sep.Batch([(c1, 3)], precision=sep.PREC_RESAMPLED, graph=graph) as b:
+            got = b.run([v1])
+            assert all(same(g, r) for g, r in zip(got, single1))
+    finally:
+        c1.close()
+        c2.close()
+
+
 def test_served_equals_single(wedges):
     with sep.Server(max_inflight=2, max_batch=4) as srv:
         ids = [srv.add_wedge(ph, sh) for _, _, ph, sh in wedges]
