@@ -508,6 +508,11 @@ size_t mw_ctx_resident_bytes(const mw_ctx* c)
             if (q) b += (q->rq.cap + q->w.cap) * sizeof(double) + q->rstep.cap * sizeof(RStep);
     }
     b += (c->star.cap + c->ratio.cap + c->kw.cap + c->kw3.cap) * sizeof(double);
+    {
+        std::lock_guard<std::mutex> lock(c->star_bg_mu);
+        for (const auto& q : c->star_bg)
+            if (q) b += q->rq.cap * sizeof(double);
+    }
     return b;
 }
 

@@ -401,7 +401,7 @@ struct mw_ctx {
     std::vector<double> conv_dx, conv_w3;
     std::vector<double> star_sin2_q;
     std::unique_ptr<mw::ResampledDev> star_bg[mw::kBgLevels];
-    std::mutex star_bg_mu;
+    mutable std::mutex star_bg_mu;
 
     // per-evaluation buffers
     mw::DevBuf<mw::EvalConst> consts;
