@@ -136,6 +136,9 @@ int build_context(mw_ctx* c, const std::vector<RegionGeom>& geoms, int n_conv, i
         c->conv_dx[i] = 5.0 * std::log10(ratio[i]);
         c->conv_w3[i] = kw[i] * (ratio[i] * ratio[i] * ratio[i]);
     }
+    double gap = 0.0;
+    for (int i = 1; i < n_conv; i++) gap = std::max(gap, std::fabs(ratio[i] - ratio[i - 1]));
+    c->star_coarse_pr = gap > 0.0 ? 2.0 / gap : 1e300;
     if (c->n_stars > 0) tabulate_star_theta(*c, soa.data(), c->n_stars);
     int rc;
     if ((rc = c->ratio.reserve(n_conv)) || (rc = c->kw.reserve(n_conv)) || (rc = c->kw3.reserve(n_conv)))
@@ -270,6 +273,7 @@ int run_plan(mw_ctx* c, Plan& P)
             L.ratio = c->ratio.p;
             L.kw = c->kw.p;
             L.kw3 = c->kw3.p;
+            L.coarse_pr = c->star_coarse_pr;
             const int sl = (P.resample && P.mixed && bg == kBgHernquist) ? star_bg_level(*c, P.K[s].q) : -1;
             if ((rc = star_bg_table(*c, sl, &L.bg_rq, &L.bg_count))) return rc;
             ReduceItem& R = c->h_sred.p[oi];

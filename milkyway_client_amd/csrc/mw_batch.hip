@@ -231,6 +231,7 @@ int BatchRunner::plan(const std::vector<Slot>& slots, int precision)
         L.ratio = c->ratio.p;
         L.kw = c->kw.p;
         L.kw3 = c->kw3.p;
+        L.coarse_pr = c->star_coarse_pr;
         int rc1;
         if ((rc1 = star_bg_table(*c, star_level(*c, slots[s].k), &L.bg_rq, &L.bg_count))) return rc1;
         hs[oi] = L;

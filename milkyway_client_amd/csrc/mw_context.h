@@ -399,6 +399,10 @@ struct mw_ctx {
     // the tabulated q values (see star_bg_level), and the background tables
     // folded onto grids uniform in the offset, built on first use.
     std::vector<double> conv_dx, conv_w3;
+    // Mixed star pass: 2 / the widest gap between adjacent ratio_i.  A
+    // stream with p r_star above it may have its Gaussian's peak between
+    // two points with both terms' exponents above 1.
+    double star_coarse_pr = 0.0;
     std::vector<double> star_sin2_q;
     std::unique_ptr<mw::ResampledDev> star_bg[mw::kBgLevels];
     mutable std::mutex star_bg_mu;

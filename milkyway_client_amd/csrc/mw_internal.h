@@ -115,7 +115,8 @@ struct StarLaunch {
     // instead of the n_conv points.
     const double* bg_rq;
     int bg_count;
-    const double* kw3;         // [n_conv] kw_i ratio_i^3 (the folded pass's stream weights / r_star^3)
+    const double* kw3;         // [n_conv] kw_i ratio_i^3 (the mixed pass's stream weights / r_star^3)
+    double coarse_pr;          // mixed pass: p r_star above which a stream's exponents are split exactly
 };
 
 // Constants of the stream separation pass (separation.c:37-254).

@@ -783,23 +783,83 @@ __global__ void reset_counters(unsigned* counters, int n)
 // Star pass
 // ---------------------------------------------------------------------------
 
+// The streams of one star over its n convolution points, mixed precision,
+// without the factor r_star^3: st[s] = sum_j kw3_j 2^-a_j(s) with
+// kw3 = kw ratio^3 (qw = r_star^3 kw3).  Along the sight t_j = (p r_star)
+// ratio_j - m and a_j = t_j^2 + e; t is monotone in ratio, so the smallest
+// t^2 over the window, t0^2, sits at an end (or is 0 where the Gaussian's
+// peak lies inside).  Each term's exponent t_j^2 - t0^2 is taken in FP64 by
+// one FMA and only then rounded to FP32, which adds a relative error below
+// (t_j^2 - t0^2) 2^-24 ln 2 to the 1-ulp error of the FP32 2^x; 2^-(e + t0^2)
+// multiplies each sum once, in FP64.  The terms that carry the sum then have
+// exponents below 1, unless the peak lies inside the window between two
+// points further apart than 2 in t (p r_star above coarse_pr): a wave with
+// such a stream on any lane splits every exponent into an integer k and a
+// fraction f in FP64 instead, 2^-k by ldexp and only 2^-f through FP32.
+template <int S>
+__device__ __forceinline__ void star_streams_shifted(const Sight<S>& sight, double rstar, int n,
+                                                     const __attribute__((address_space(4))) double* ratio,
+                                                     const __attribute__((address_space(4))) double* kw3,
+                                                     double coarse_pr, double (&st)[kMaxStreams])
+{
+    double pr[S > 0 ? S : 1], t0sq[S > 0 ? S : 1];
+    bool coarse = false;
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        pr[s] = sight.st[s].p * rstar;
+        const double ta = fma(pr[s], ratio[0], -sight.st[s].m);
+        const double tb = fma(pr[s], ratio[n - 1], -sight.st[s].m);
+        const double lo = fmin(ta, tb), hi = fmax(ta, tb);
+        const double t0 = lo > 0.0 ? lo : (hi < 0.0 ? hi : 0.0);
+        t0sq[s] = t0 * t0;
+        coarse |= t0 == 0.0 && pr[s] > coarse_pr;
+    }
+    if (__any(coarse)) {
+        for (int j = 0; j < n; j++) {
+            const double rho = ratio[j], w = kw3[j];
+#pragma unroll
+            for (int s = 0; s < S; s++) {
+                const double t = fma(pr[s], rho, -sight.st[s].m);
+                const double a = fma(t, t, -t0sq[s]);
+                const double k = floor(a);
+                const float e = __builtin_amdgcn_exp2f(-(float)(a - k));
+                st[s] = fma(w, ldexp((double)e, -(int)fmin(k, 2000.0)), st[s]);
+            }
+        }
+    } else {
+        for (int j = 0; j < n; j++) {
+            const double rho = ratio[j], w = kw3[j];
+#pragma unroll
+            for (int s = 0; s < S; s++) {
+                const double t = fma(pr[s], rho, -sight.st[s].m);
+                const float e = __builtin_amdgcn_exp2f(-(float)fma(t, t, -t0sq[s]));
+                st[s] = fma(w, (double)e, st[s]);
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < S; s++) st[s] *= exp2_neg(sight.st[s].e + t0sq[s]);
+}
+
 // Convolved densities of star i (evaluation_optimized.c:190-299 for one star):
 // bg is the background density times reff(r) 3 sigma / r^3 (or -1 when
 // q == 0); st[s] is the stream density without that factor and without the
 // stream's on/off flag.
 //
-// bg_count > 0 (resampled mixed pass, Hernquist profile): the background
-// term from the bg_count (r / r_star, weight / r_star^3) pairs of bg_tab, a
-// fold of the convolution onto a grid uniform in the magnitude offset; the
-// n points then carry the streams alone.
+// Mixed precision (Hernquist profile): the streams by star_streams_shifted
+// (conv_kw3 = kw ratio^3), the background at the n points or, with
+// bg_count > 0 (resampled pass), from the bg_count (r / r_star,
+// weight / r_star^3) pairs of bg_tab, a fold of the convolution onto a grid
+// uniform in the magnitude offset.
 template <int S, int BG, bool MIXED>
 __device__ __forceinline__ void star_densities(const double* __restrict__ star, long N, long i, int n,
                                                const EvalConst& K,
                                                const double* __restrict__ conv_ratio,
                                                const double* __restrict__ conv_kw, double& reff,
                                                double& bg, double (&st)[kMaxStreams],
+                                               const double* __restrict__ conv_kw3, double coarse_pr,
                                                const double* __restrict__ bg_tab = nullptr,
-                                               int bg_count = 0, const double* __restrict__ conv_kw3 = nullptr)
+                                               int bg_count = 0)
 {
     const Sight<S> sight = make_sight<S, BG, MIXED>(star[i], star[N + i], star[2 * N + i], K);
     const double rstar = star[3 * N + i];
@@ -812,25 +872,25 @@ __device__ __forceinline__ void star_densities(const double* __restrict__ star, 
     // constant address space, where uniform loads become scalar loads.
     typedef const __attribute__((address_space(4))) double* ConstPtr;
     const ConstPtr ratio = (ConstPtr)conv_ratio, kw = (ConstPtr)conv_kw;
-    bool folded = false;
     if constexpr (MIXED && BG == kBgHernquist) {
+        const double rs3 = rstar * rstar * rstar;
+        star_streams_shifted<S>(sight, rstar, n, ratio, (ConstPtr)conv_kw3, coarse_pr, st);
+#pragma unroll
+        for (int s = 0; s < S; s++) st[s] *= rs3;
+        double b = 0.0;
         if (bg_count > 0) {
-            folded = true;
-            double unused = 0.0;
-            const ConstPtr kw3 = (ConstPtr)conv_kw3;   // kw ratio^3: qw = r_star^3 kw3
-            const double rs3 = rstar * rstar * rstar;
-            for (int j = 0; j < n; j++) {
-                const double r = rstar * ratio[j];
-                point_terms<S, kBgZeroQ, MIXED>(r, rs3 * kw3[j], sight, K, unused, st);
-            }
             const ConstPtr bq = (ConstPtr)bg_tab;
-            double b = 0.0;
 #pragma unroll 2
             for (int m = 0; m < bg_count; m++) bg_point<BG>(rstar * bq[2 * m], bq[2 * m + 1], sight.bg, K, b);
-            bg = b * (rstar * rstar * rstar);
+            b *= rs3;
+        } else {
+            for (int j = 0; j < n; j++) {
+                const double r = rstar * ratio[j];
+                bg_point<BG>(r, kw[j] * (r * r * r), sight.bg, K, b);
+            }
         }
-    }
-    if (!folded) {
+        bg = b;
+    } else {
         for (int j = 0; j < n; j++) {
             const double r = rstar * ratio[j];
             const double qw = kw[j] * (r * r * r);
@@ -869,8 +929,8 @@ __device__ __forceinline__ void star_block(const StarLaunch& it, long lb,
 
     double reff, bg;
     double st[kMaxStreams];
-    star_densities<S, BG, MIXED>(it.star, N, i, it.n_conv, K, conv_ratio, conv_kw, reff, bg, st, it.bg_rq,
-                                 it.bg_count, it.kw3);
+    star_densities<S, BG, MIXED>(it.star, N, i, it.n_conv, K, conv_ratio, conv_kw, reff, bg, st, it.kw3,
+                                 it.coarse_pr, it.bg_rq, it.bg_count);
 
     // Mixture (evaluation_optimized.c:674-716).
     DD out[kStarComps];
@@ -1025,7 +1085,7 @@ __global__ __launch_bounds__(kBlock) void separation_kernel(StarLaunch it, EvalC
 
     double reff, bg;
     double st[kMaxStreams];
-    star_densities<S, BG, MIXED>(it.star, N, i, it.n_conv, K, it.ratio, it.kw, reff, bg, st);
+    star_densities<S, BG, MIXED>(it.star, N, i, it.n_conv, K, it.ratio, it.kw, reff, bg, st, it.kw3, it.coarse_pr);
 
     // Membership probabilities (separation.c:170-189).
     const double pbx = C.eps_b * bg / C.I_bg;

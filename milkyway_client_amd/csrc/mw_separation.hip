@@ -107,6 +107,8 @@ int mw_ctx_separation(mw_ctx* c, const double* params, int precision, const doub
         it.n_streams = S;
         it.ratio = c->ratio.p;
         it.kw = c->kw.p;
+        it.kw3 = c->kw3.p;
+        it.coarse_pr = c->star_coarse_pr;
         const bool mixed = precision == MW_PREC_MIXED;
         if ((rc = c->ensure_stream())) return rc;
         MW_HIP_CHECK(launch_separation(S, K.bg_mode, mixed, it, K, C, c->sep_assign.p, c->sep_xyz.p,

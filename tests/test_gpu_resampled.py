@@ -471,3 +471,20 @@ def test_served_equals_single(wedges):
 def test_drop_in_does_not_take_resampled():
     with pytest.raises(sep.MwError):
         sep.set_precision(sep.PREC_RESAMPLED)
+
+
+def test_narrow_stream_between_convolution_points():
+    """A 0.5 kpc stream seen through 30 convolution points.  The mixed star
+    pass rounds each stream exponent, shifted by its smallest value over the
+    star's window, to FP32; where the Gaussian's peak lies inside the window
+    between points more than 2 apart in t (p r_star above 2 / the widest
+    ratio gap, 17.8 at 30 points: with k = 1.7 per kpc here, any star beyond
+    ~11 kpc on a sight across the stream), every exponent there may be large,
+    and the kernel splits them exactly instead.  Both paths run in this
+    workload; every star stays within the bar."""
+    shape = synth.WedgeShape(700, 24, 16, 30, 2000)
+    p, stars = synth.workload(141, shape, 2)
+    p.streams[0].parameters[4] = 0.5
+    e, grew = errors_vs_oracle(p, stars)
+    assert e["lnP"] <= LNP_TOL and e["lnL"] <= LNP_TOL
+    assert e["I_bg"] <= INT_REL_TOL and e["I_st"] <= INT_REL_TOL
