@@ -325,8 +325,9 @@ def bench_batch(args):
     quad_desc = ("resampled: each region's r-step x convolution weights folded by 16-node Lagrange "
                  "interpolation onto grids sized to the model's features (MW_PREC_RESAMPLED): each "
                  "stream's Gaussian by FP64 recurrence on a distance grid of its own (rows it does "
-                 "not reach skipped), the background on a magnitude grid of its own; star pass "
-                 "exact" if quad == "resampled" else
+                 "not reach skipped), the background on a magnitude grid of its own; star pass: "
+                 "streams at every convolution point, background folded over the convolution "
+                 "offsets" if quad == "resampled" else
                  "exact: the reference's r-step x convolution points (MW_PREC_MIXED)")
     out = {
         "metric": "likelihood evals/s, batched mixed precision vs plain FP64 @ per-star |d lnP| <= 1e-7",
