@@ -565,16 +565,18 @@ __device__ __forceinline__ void integral_block_split(const RegionLaunch& it, lon
                 min_t2(fma(p, lo, -mm), fma(p, fma((double)R * n - 1.0, h, lo), -mm)) + kNegligibleLog2;
             const double T = floor_t2 * rsqrt_nr(floor_t2);
             const double dn = d * n, d1 = d * (n - 1);
-            // Four terms per step, in Horner form:
-            //   a += G_j (w_j + R_j (w_{j+1} + R_{j+1} (w_{j+2} + R_{j+2} w_{j+3})))
-            // with R_{j+1} = R_j c and R_{j+2} = R_j c^2; then G_{j+4} = G_j P_j,
-            // P_j = R_j R_{j+1} R_{j+2} R_{j+3} = R_j^4 c^6, P_{j+4} = P_j c^16 and
-            // R_{j+4} = R_j c^4.  10 VALU per four terms instead of 12 (the
-            // innermost w_{j+2} + R w_{j+3} takes two: one instruction reads at
-            // most one SGPR pair).  c^16 by squarings carries about 15 ulp, so
-            // after m steps of four since a restart the error stays below
-            // 15 m^2 / 2 ulp: 3.4e-12 relative at the restart interval's 64.
-            const double c4 = (c * c) * (c * c), c16 = (c4 * c4) * (c4 * c4);
+            // Eight terms per step, in Horner form:
+            //   a += G_j (w_j + R_j (w_{j+1} + R_{j+1} (... + R_{j+6} w_{j+7})))
+            // with R_{j+k} = R_j c^k; then G_{j+8} = G_j P_j,
+            // P_j = R_j R_{j+1} ... R_{j+7} = R_j^8 c^28, P_{j+8} = P_j c^64 and
+            // R_{j+8} = R_j c^8.  18 VALU per eight terms instead of 24 (the
+            // innermost w_{j+6} + R w_{j+7} takes two: one instruction reads at
+            // most one SGPR pair; four terms per step took 20).  c^64 by
+            // squarings carries about 63 ulp, so after m steps since a restart
+            // the error stays below 63 m^2 / 2 ulp: 3.6e-12 relative at the
+            // restart interval's 32.
+            const double c4 = (c * c) * (c * c), c8 = c4 * c4, c16 = c8 * c8, c24 = c16 * c8,
+                         c64 = (c16 * c16) * (c16 * c16);
             const ConstPtr wq = (ConstPtr)(it.st_rq[s]);
             double G = 0.0, Rr = 0.0, Pq = 0.0, a = 0.0;
             double t0 = fma(p, lo, -mm);
@@ -589,20 +591,24 @@ __device__ __forceinline__ void integral_block_split(const RegionLaunch& it, lon
                     const double tc = fmax(t0, -40.0);
                     G = exp2_neg(tc * tc);
                     Rr = exp2_neg(fma(2.0 * tc, d, d * d));
-                    const double q = Rr * Rr * c;
-                    Pq = q * q * c4;   // (R^2 c)^2 c^4 = R^4 c^6
+                    const double q = Rr * Rr * c, q2 = q * q;
+                    Pq = (q2 * q2) * c24;   // (R^2 c)^4 c^24 = R^8 c^28
                     since = 0;
                 }
                 const ConstPtr row = wq + (long)k * n;
-#pragma unroll 2
-                for (int i = 0; i < n; i += 4) {
-                    const double r1 = Rr * c, r2 = r1 * c;
-                    const double x = mul_add_sgpr(r2, row[i + 3], row[i + 2]);
-                    const double y = fma_sgpr(r1, x, row[i + 1]);
-                    a = fma(G, fma_sgpr(Rr, y, row[i]), a);
+#pragma unroll 1
+                for (int i = 0; i < n; i += 8) {
+                    const double r1 = Rr * c, r2 = r1 * c, r3 = r2 * c, r4 = r3 * c, r5 = r4 * c, r6 = r5 * c;
+                    double x = mul_add_sgpr(r6, row[i + 7], row[i + 6]);
+                    x = fma_sgpr(r5, x, row[i + 5]);
+                    x = fma_sgpr(r4, x, row[i + 4]);
+                    x = fma_sgpr(r3, x, row[i + 3]);
+                    x = fma_sgpr(r2, x, row[i + 2]);
+                    x = fma_sgpr(r1, x, row[i + 1]);
+                    a = fma(G, fma_sgpr(Rr, x, row[i]), a);
                     G *= Pq;
-                    Pq *= c16;
-                    Rr *= c4;
+                    Pq *= c64;
+                    Rr *= c8;
                 }
             }
             acc[1 + s] = a * exp2_neg(sight.st[s].e);   // the sight's minimum, taken out of every term

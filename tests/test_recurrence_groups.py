@@ -1,40 +1,46 @@
-"""The split layout's stream recurrence, four terms per step, pinned on the
+"""The split layout's stream recurrence, eight terms per step, pinned on the
 CPU in FP64 (numpy), in the kernel's order of operations.
 
 Along a distance grid t_j = t_0 + j d, the terms G_j = 2^-(t_j^2) obey
 G_{j+1} = G_j R_j and R_{j+1} = R_j c with c = 2^-(2 d^2).  The kernel takes
-four terms per step in Horner form,
-    a += G_j (w_j + R_j (w_{j+1} + R_{j+1} (w_{j+2} + R_{j+2} w_{j+3}))),
-then G_{j+4} = G_j P_j with P_j = R_j^4 c^6, P_{j+4} = P_j c^16 and
-R_{j+4} = R_j c^4, restarting from the exact 2^x every 8 rows of 32 nodes.
-Over that interval the sum must stay within 4e-12 of the direct one for any
-start and spacing the kernel allows (d <= 1/2, t_0 >= -40); a start P_0 of
-R^4 c^4 (one factor c^2 short) must fail.
+eight terms per step in Horner form,
+    a += G_j (w_j + R_j (w_{j+1} + R_{j+1} (... + R_{j+6} w_{j+7}))),
+R_{j+k} = R_j c^k, then G_{j+8} = G_j P_j with P_j = R_j^8 c^28,
+P_{j+8} = P_j c^64 and R_{j+8} = R_j c^8, restarting from the exact 2^x every
+8 rows of 32 nodes.  Over that interval the sum must stay within 4e-12 of
+the direct one for any start and spacing the kernel allows (d <= 1/2,
+t_0 >= -40); a start P_0 of R^8 c^24 (one factor c^4 short) must fail.
 """
 import numpy as np
 
 ROWS, N = 8, 32
 
 
-def grouped(w, t0, d, p0_power=6):
+def grouped(w, t0, d, short_start=False):
     c = 2.0 ** (-2 * d * d)
     c4 = (c * c) * (c * c)
-    c16 = (c4 * c4) * (c4 * c4)
+    c8 = c4 * c4
+    c16 = c8 * c8
+    c24 = c16 * c8
+    c64 = (c16 * c16) * (c16 * c16)
     tc = max(t0, -40.0)
     G = 2.0 ** (-tc * tc)
     R = 2.0 ** (-(2 * tc * d + d * d))
     q = R * R * c
-    P = q * q * (c4 if p0_power == 6 else c * c)
+    q2 = q * q
+    P = (q2 * q2) * (c16 * c4 if short_start else c24)
     a = 0.0
-    for i in range(0, w.size, 4):
-        r1 = R * c
-        r2 = r1 * c
-        x = r2 * w[i + 3] + w[i + 2]
-        y = r1 * x + w[i + 1]
-        a += G * (R * y + w[i])
+    for i in range(0, w.size, 8):
+        r = [R]
+        for _ in range(6):
+            r.append(r[-1] * c)
+        x = r[6] * w[i + 7] + w[i + 6]
+        for k in range(5, -1, -1):
+            x = r[k] * x + w[i + k]
+        a += G * x
         G *= P
-        P *= c16
-        R *= c4
+        P *= c64
+        R *= c8
     return a
 
 
@@ -59,4 +65,4 @@ def test_grouped_recurrence_bound():
 
 
 def test_grouped_recurrence_catches_a_short_start():
-    assert worst_error(p0_power=4) > 1e-3
+    assert worst_error(short_start=True) > 1e-3
