@@ -189,7 +189,9 @@ __device__ __forceinline__ double rsqrt_nr(double a)
 
 // a b + c with c uniform in an SGPR pair, as one VOP3 v_fma_f64 (the
 // compiler's own choice, the two-address v_fmac_f64, first copies c into a
-// VGPR pair with two moves).
+// VGPR pair with two moves).  c must be the same on every lane of the wave
+// (here: table entries read by scalar loads); the "s" operand holds one value
+// per wave.
 __device__ __forceinline__ double fma_sgpr(double a, double b, double c)
 {
     double r;
@@ -197,8 +199,9 @@ __device__ __forceinline__ double fma_sgpr(double a, double b, double c)
     return r;
 }
 
-// a b + c with b and c uniform in SGPR pairs: one instruction reads at most
-// one SGPR pair, so a multiply and an add.
+// a b + c with b and c uniform in SGPR pairs (the same on every lane, as
+// above): one instruction reads at most one SGPR pair, so a multiply and an
+// add.
 __device__ __forceinline__ double mul_add_sgpr(double a, double b, double c)
 {
     double r;
