@@ -91,8 +91,6 @@ def build(verbose: bool = False) -> Path:
         _run(["gcc", *GCC_FLAGS, *inc, str(CSRC / src), "-o", str(tmp), f"-L{LIB_DIR}",
               "-lmw_separation_hip", "-Wl,-rpath,$ORIGIN"])
         os.replace(tmp, LIB_DIR / name)
-    # Byte-counter calibration probe (a plain copy kernel of known size).
-    _run([hipcc, *HIPCC_FLAGS, str(CSRC / "hbm_copy_probe.hip"), "-o", str(LIB_DIR / "hbm_copy_probe")])
     return LIB_PATH
 
 

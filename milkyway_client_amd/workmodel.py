@@ -28,17 +28,6 @@ def flops_per_point(n_streams: int) -> int:
     return BG_OPS + W_SQRT + W_DIV + n_streams * (STREAM_OPS + W_EXP)
 
 
-# On a grid uniform in distance the resampled kernels take each stream's
-# Gaussian by a multiplicative recurrence instead of evaluating it: per node
-# and stream the term's multiply-add and the two recurrence multiplies, with
-# no exponential.  Same weights, counted on the operations that method runs.
-STREAM_RECURRENCE_OPS = 4
-
-
-def flops_per_point_recurrence(n_streams: int) -> int:
-    return BG_OPS + W_SQRT + W_DIV + n_streams * STREAM_RECURRENCE_OPS
-
-
 def integral_points(regions, n_conv: int) -> int:
     """Convolution points of the integral over every region (r x mu x nu x n)."""
     return sum(g.r_steps * g.mu_steps * g.nu_steps for g in regions) * n_conv

@@ -73,3 +73,26 @@ def test_bench_reads_measured_traffic():
     import bench
     assert bench.measured_traffic("integral_kernel_fp64_full") > 36_000_000
     assert bench.measured_traffic("no_such_kernel") is None
+    assert bench.measured_counters("no_such_kernel") is None
+
+
+def test_bench_cpu_sample_keeps_the_cells():
+    """The CPU baseline's sample keeps each region's first mu rows with the
+    same cell size, so that its cost per row is the full wedge's."""
+    import bench
+    p, _ = synth.workload(1, synth.FULL_SIZE, 3)
+    q = bench.first_rows(p, 16)
+    for a, b in zip(p.regions, q.regions):
+        assert b.mu_steps == 16 and b.mu_min == a.mu_min
+        assert (b.mu_max - b.mu_min) / b.mu_steps == pytest.approx((a.mu_max - a.mu_min) / a.mu_steps)
+        assert (b.r_steps, b.nu_steps, b.r_min, b.r_max) == (a.r_steps, a.nu_steps, a.r_min, a.r_max)
+    assert p.regions[0].mu_steps == 1600          # the original is untouched
+
+
+def test_bench_vector_sets_are_fresh():
+    import bench
+    plist = [synth.workload(100 + k, s, 4) for k, s in enumerate(synth.BATCH_SHAPES[:2])]
+    a, b = bench.vector_set(0, plist), bench.vector_set(1, plist)
+    assert all(x.shape == (16, plist[0][0].optimized_parameter_count()) for x in a)
+    assert not any(np.array_equal(x, y) for x, y in zip(a, b))
+    assert all(np.array_equal(x, y) for x, y in zip(a, bench.vector_set(0, plist)))
