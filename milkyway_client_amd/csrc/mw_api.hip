@@ -91,6 +91,19 @@ int build_context(mw_ctx* c, const std::vector<RegionGeom>& geoms, int n_conv, i
         if ((rc = rd->rstep.reserve(t.rstep.size()))) return rc;
         if ((rc = rd->col.reserve(t.col.size()))) return rc;
         MW_HIP_CHECK(hipMemcpy(rd->rq.p, t.rq.data(), sizeof(double) * t.rq.size(), hipMemcpyHostToDevice));
+        {
+            std::vector<double> q32(t.rq.size());
+            for (size_t i = 0; i + 1 < t.rq.size(); i += 2) {
+                q32[i] = t.rq[i];
+                const float w = (float)t.rq[i + 1];
+                uint32_t bits;
+                std::memcpy(&bits, &w, 4);
+                const uint64_t word = bits;   // qw in the low half, 0 in the high half
+                std::memcpy(&q32[i + 1], &word, 8);
+            }
+            if ((rc = rd->rq32.reserve(q32.size()))) return rc;
+            MW_HIP_CHECK(hipMemcpy(rd->rq32.p, q32.data(), sizeof(double) * q32.size(), hipMemcpyHostToDevice));
+        }
         MW_HIP_CHECK(hipMemcpy(rd->rstep.p, t.rstep.data(), sizeof(RStep) * t.rstep.size(), hipMemcpyHostToDevice));
         MW_HIP_CHECK(hipMemcpy(rd->col.p, t.col.data(), sizeof(double) * t.col.size(), hipMemcpyHostToDevice));
         // What the resampled quadrature needs: the exact tables on the host
@@ -193,7 +206,8 @@ int run_plan(mw_ctx* c, Plan& P)
             const int level = gp.level, layout = gp.layout;
             if ((rc = region_tables(rd, gp, &tr))) return rc;
             RegionLaunch& L = c->h_ritems.p[item];
-            L.rq = tr.rq;
+            L.qw_f32 = (P.mixed && level < 0 && bg != kBgGeneral) ? 1 : 0;
+            L.rq = L.qw_f32 ? rd.rq32.p : tr.rq;
             L.rstep = tr.rstep;
             L.col = rd.col.p;
             L.n_cols = rd.n_cols;
@@ -503,7 +517,7 @@ size_t mw_ctx_resident_bytes(const mw_ctx* c)
     if (!c) return 0;
     size_t b = 0;
     for (const auto& r : c->regions) {
-        b += r->rq.cap * sizeof(double) + r->rstep.cap * sizeof(RStep) + r->col.cap * sizeof(double);
+        b += (r->rq.cap + r->rq32.cap + r->col.cap) * sizeof(double) + r->rstep.cap * sizeof(RStep);
         std::lock_guard<std::mutex> lock(r->level_mu);
         for (const auto& per_level : r->levels)
             for (const auto& q : per_level)

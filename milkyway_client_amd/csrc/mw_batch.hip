@@ -131,7 +131,8 @@ int BatchRunner::plan(const std::vector<Slot>& slots, int precision)
             points_ += rd->n_cols * tr.nodes;
             RegionLaunch L;
             std::memset(&L, 0, sizeof(L));
-            L.rq = tr.rq;
+            L.qw_f32 = (mixed_ && level < 0 && slots[s].k.bg_mode != kBgGeneral) ? 1 : 0;
+            L.rq = L.qw_f32 ? rd->rq32.p : tr.rq;
             L.rstep = tr.rstep;
             L.col = rd->col.p;
             L.n_cols = rd->n_cols;

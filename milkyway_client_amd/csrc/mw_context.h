@@ -281,6 +281,9 @@ struct RegionDev {
     long n_cols = 0;
     int blocks = 0;
     DevBuf<double> rq, col;
+    // rq with FP32 weights for the mixed exact kernels: per point the FP64
+    // distance, then qw rounded to FP32 in the low half of the second word
+    DevBuf<double> rq32;
     DevBuf<RStep> rstep;
     // Host copies of the exact r-step tables, and what the choice of a
     // resampled grid needs: the largest convolution distance (kpc), the

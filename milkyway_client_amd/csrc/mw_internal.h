@@ -96,6 +96,9 @@ struct RegionLaunch {
     const double* st_rq[kMaxStreams];
     double st_step[kMaxStreams], st_lo[kMaxStreams];
     int st_rows[kMaxStreams];
+    // 1: rq is the exact tables' copy with FP32 weights (16 bytes per point:
+    // the FP64 distance, then qw in FP32), read by the mixed kernels' FP32 rows
+    int qw_f32;
 };
 
 // Stars resident on the device, SoA.

@@ -332,6 +332,60 @@ __device__ __forceinline__ void bg_point(double r, double qw, const Quad& q, con
     }
 }
 
+// The Hernquist background at one point in FP32, for the mixed exact
+// quadrature: qw / (rg (rg + r0)^3) with rg^2 = t^2 + e, t = p r - m taken in
+// FP64 (the shifted quadratic loses nothing to cancellation) and rounded
+// once.  1/rg is v_rsq_f32; 1/(rg + r0) is v_rcp_f32 refined by one FP32
+// Newton step.  Measured on gfx950 over whole binades, v_rsq_f32 errs by
+// at most 0.87 ulp with a mean of -0.08 ulp and v_rcp_f32 by 0.88 ulp with a
+// mean of -0.07 ulp, which the Newton step takes to 0.5 ulp and a mean of
+// 0.0001 ulp; v_sqrt_f32 (mean -0.15 ulp, amplified up to four times by
+// the cube) is not used.  What remains systematic is the rsq's bias, which
+// enters the term as b (3 rg / (rg + r0) - 1), |b| <= 0.08 ulp: below 2e-8
+// relative in the background integral.  The term is added to a per-row
+// FP32 sum (at most 1024 positive terms, rounded to nearest).
+__device__ __forceinline__ void bg_point_f32(double r, float qw, const Quad& q, float r0, float e,
+                                             float& bg)
+{
+    const float t = (float)fma(q.p, r, -q.m);
+    const float rg2 = fmaf(t, t, e);
+    const float y = __builtin_amdgcn_rsqf(rg2);
+    const float s = fmaf(rg2, y, r0);
+    float z = __builtin_amdgcn_rcpf(s);
+    z = fmaf(z, fmaf(-s, z, 1.0f), z);
+    bg = fmaf(qw * y, z * z * z, bg);
+}
+
+// One r-step row of the mixed exact quadrature: the background (Hernquist
+// profile) by bg_point_f32 and each stream's 2^-(t^2) by v_exp_f32, t in
+// FP64, every term weighted by the FP32 qw and summed per row in FP32; the
+// row sums are returned in FP64, where the caller scales them and adds them
+// to the column's Kahan sums.  Row entries are 16 bytes: the FP64 distance,
+// then qw as an FP32 value in the low half of the second word.
+template <int S, int BG, int UNROLL>
+__device__ __forceinline__ void mixed_row_f32(const double2* row, int n, const Sight<S>& v, float r0,
+                                              float e, double& bg, double (&st)[kMaxStreams])
+{
+    float b = 0.0f;
+    float a[S > 0 ? S : 1];
+#pragma unroll
+    for (int s = 0; s < (S > 0 ? S : 1); s++) a[s] = 0.0f;
+#pragma unroll UNROLL
+    for (int i = 0; i < n; i++) {
+        const double2 en = row[i];
+        const float qw = __uint_as_float((unsigned)__double_as_longlong(en.y));
+        if constexpr (BG != kBgZeroQ) bg_point_f32(en.x, qw, v.bg, r0, e, b);
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            const double t = fma(v.st[s].p, en.x, -v.st[s].m);
+            a[s] = fmaf(qw, __builtin_amdgcn_exp2f(-(float)(t * t)), a[s]);
+        }
+    }
+    bg = (double)b;
+#pragma unroll
+    for (int s = 0; s < S; s++) st[s] = (double)a[s];
+}
+
 template <int S, int BG, bool MIXED, bool SHIFTED = false>
 __device__ __forceinline__ void point_terms(double r, double qw, const Sight<S>& v,
                                             const EvalConst& K, double& bg, double (&st)[kMaxStreams])
@@ -674,6 +728,10 @@ __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
         double acc[1 + S], cmp[1 + S];
 #pragma unroll
         for (int c = 0; c <= S; c++) acc[c] = cmp[c] = 0.0;
+        // FP32 rows (mixed, exact tables, Hernquist profile): the table's
+        // weights are FP32 and the per-row sums run in FP32.
+        const bool f32_rows = MIXED && BG != kBgGeneral && it.qw_f32;
+        const float r0f = (float)K.r0, ebf = (float)sight.bg.e;
 
         const int rt = kTileD2 / n;   // r-steps per LDS tile (host guarantees n <= kTileD2)
         const double2* rq = reinterpret_cast<const double2*>(it.rq);
@@ -687,10 +745,22 @@ __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
                 double bg = 0.0;
                 double st[kMaxStreams] = { 0.0, 0.0, 0.0, 0.0 };
                 const double2* row = tile + kk * n;
+                if constexpr (MIXED && BG != kBgGeneral) {
+                    if (f32_rows) {
+                        mixed_row_f32<S, BG, UNROLL>(row, n, sight, r0f, ebf, bg, st);
+                    } else {
 #pragma unroll UNROLL
-                for (int i = 0; i < n; i++) {
-                    const double2 p = row[i];
-                    point_terms<S, BG, MIXED, MIXED>(p.x, p.y, sight, K, bg, st);
+                        for (int i = 0; i < n; i++) {
+                            const double2 p = row[i];
+                            point_terms<S, BG, MIXED, MIXED>(p.x, p.y, sight, K, bg, st);
+                        }
+                    }
+                } else {
+#pragma unroll UNROLL
+                    for (int i = 0; i < n; i++) {
+                        const double2 p = row[i];
+                        point_terms<S, BG, MIXED, MIXED>(p.x, p.y, sight, K, bg, st);
+                    }
                 }
                 const RStep& rs = it.rstep[k];
                 double f = rs.reff_xr_rp3 * rs.irv;
