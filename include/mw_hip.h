@@ -14,8 +14,14 @@
  *    evaluates many parameter vectors per call.  It returns status codes and
  *    never exits.
  *
- * Limits of this build: at most MW_MAX_STREAMS streams per model and at most
- * MW_MAX_CONVOLVE convolution points.
+ * Limits of this build: the drop-in entry points take any number of streams
+ * (as the reference's calculate_integrals and calculate_likelihood do);
+ * contexts take at most MW_MAX_STREAMS streams per model (the size of
+ * mw_result's arrays), batches and the request server at most 4, and the
+ * stream separation 1 to 4 (the reference's prob_ok, probability.c:310-403,
+ * handles 1 to 4).  Models of more than 4 streams are evaluated in passes of
+ * 4 streams on the exact quadrature.  At most MW_MAX_CONVOLVE convolution
+ * points.
  */
 #ifndef MW_HIP_H
 #define MW_HIP_H
@@ -28,7 +34,7 @@
 extern "C" {
 #endif
 
-#define MW_MAX_STREAMS 4
+#define MW_MAX_STREAMS 16
 #define MW_MAX_CONVOLVE 1024
 
 /* Status codes */

@@ -56,11 +56,15 @@ struct PlanRegion {
 
 struct Plan {
     std::vector<EvalConst> K;              // one per slot
+    // Slots per parameter vector: a model of S streams takes stream_groups(S)
+    // consecutive slots (vector v's group g is slot v * groups + g), whose
+    // star passes run in group order and chain each star's mixture sum.
+    int groups = 1;
     std::vector<PlanRegion> regions;       // slot-major, region order within a slot
     bool integrals = false, stars = false, mixed = false, device_finalize = false;
     bool resample = false;                 // resampled quadrature where it pays (whole regions)
     long star_begin = 0;
-    double* log10p_host = nullptr;         // [n_slots][n_stars]
+    double* log10p_host = nullptr;         // [n_slots / groups][n_stars]
 };
 
 int build_context(mw_ctx* c, const std::vector<RegionGeom>& geoms, int n_conv, int wedge, int sgr,
@@ -91,19 +95,6 @@ int build_context(mw_ctx* c, const std::vector<RegionGeom>& geoms, int n_conv, i
         if ((rc = rd->rstep.reserve(t.rstep.size()))) return rc;
         if ((rc = rd->col.reserve(t.col.size()))) return rc;
         MW_HIP_CHECK(hipMemcpy(rd->rq.p, t.rq.data(), sizeof(double) * t.rq.size(), hipMemcpyHostToDevice));
-        {
-            std::vector<double> q32(t.rq.size());
-            for (size_t i = 0; i + 1 < t.rq.size(); i += 2) {
-                q32[i] = t.rq[i];
-                const float w = (float)t.rq[i + 1];
-                uint32_t bits;
-                std::memcpy(&bits, &w, 4);
-                const uint64_t word = bits;   // qw in the low half, 0 in the high half
-                std::memcpy(&q32[i + 1], &word, 8);
-            }
-            if ((rc = rd->rq32.reserve(q32.size()))) return rc;
-            MW_HIP_CHECK(hipMemcpy(rd->rq32.p, q32.data(), sizeof(double) * q32.size(), hipMemcpyHostToDevice));
-        }
         MW_HIP_CHECK(hipMemcpy(rd->rstep.p, t.rstep.data(), sizeof(RStep) * t.rstep.size(), hipMemcpyHostToDevice));
         MW_HIP_CHECK(hipMemcpy(rd->col.p, t.col.data(), sizeof(double) * t.col.size(), hipMemcpyHostToDevice));
         // What the resampled quadrature needs: the exact tables on the host
@@ -179,7 +170,10 @@ int run_plan(mw_ctx* c, Plan& P)
     std::memcpy(c->h_consts.p, P.K.data(), sizeof(EvalConst) * n_slots);
 
     // ---- integral work items, grouped by kernel variant ----------------------
-    struct Group { int bg; int first_item, n_items, first_block, n_blocks; };
+    // Launch groups: items of one background mode and one stream count (the
+    // kernels' template parameters); star groups also of one stream group,
+    // launched in group order.
+    struct Group { int bg, S, chain; int first_item, n_items, first_block, n_blocks; };
     std::vector<Group> igroups;
     int total_iblocks = 0;
     long points = 0, points_rec = 0, points_bg = 0, stream_terms = 0;
@@ -191,14 +185,17 @@ int run_plan(mw_ctx* c, Plan& P)
         std::vector<int> order(n_pr);
         for (int i = 0; i < n_pr; i++) order[i] = i;
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-            return P.K[P.regions[a].slot].bg_mode < P.K[P.regions[b].slot].bg_mode;
+            const EvalConst &ka = P.K[P.regions[a].slot], &kb = P.K[P.regions[b].slot];
+            if (ka.bg_mode != kb.bg_mode) return ka.bg_mode < kb.bg_mode;
+            return ka.n_streams < kb.n_streams;
         });
         int item = 0;
         for (int oi = 0; oi < n_pr; oi++) {
             const PlanRegion& pr = P.regions[order[oi]];
             const int bg = P.K[pr.slot].bg_mode;
-            if (igroups.empty() || igroups.back().bg != bg)
-                igroups.push_back(Group{ bg, item, 0, total_iblocks, 0 });
+            const int nS = P.K[pr.slot].n_streams;
+            if (igroups.empty() || igroups.back().bg != bg || igroups.back().S != nS)
+                igroups.push_back(Group{ bg, nS, 0, item, 0, total_iblocks, 0 });
             Group& g = igroups.back();
             RegionDev& rd = *c->regions[pr.region];
             TableRef tr;
@@ -206,8 +203,7 @@ int run_plan(mw_ctx* c, Plan& P)
             const int level = gp.level, layout = gp.layout;
             if ((rc = region_tables(rd, gp, &tr))) return rc;
             RegionLaunch& L = c->h_ritems.p[item];
-            L.qw_f32 = (P.mixed && level < 0 && bg != kBgGeneral) ? 1 : 0;
-            L.rq = L.qw_f32 ? rd.rq32.p : tr.rq;
+            L.rq = tr.rq;
             L.rstep = tr.rstep;
             L.col = rd.col.p;
             L.n_cols = rd.n_cols;
@@ -228,7 +224,7 @@ int run_plan(mw_ctx* c, Plan& P)
             L.out_index = order[oi];
             L.block_begin = g.n_blocks;          // block index within the group's launch
             L.block_count = rd.blocks;
-            L.n_streams = c->S;
+            L.n_streams = nS;
             L.grid_step = layout > 0 ? recurrence_step(level, P.K[pr.slot]) : 0.0;
             L.grid_lo = tr.lo;
             L.bg_stride = layout_bg_stride(layout);
@@ -237,7 +233,7 @@ int run_plan(mw_ctx* c, Plan& P)
             copy_stream_tables(tr, &L);
             if (L.grid_step > 0.0) {
                 points_rec += rd.n_cols * tr.nodes;
-                stream_terms += rd.n_cols * (layout == kLayoutSplit ? tr.stream_terms : tr.nodes * c->S);
+                stream_terms += rd.n_cols * (layout == kLayoutSplit ? tr.stream_terms : tr.nodes * nS);
             }
             points_bg += (level >= 0 ? rd.n_cols * tr.bg_nodes : (pr.end - pr.begin) * (long)rd.n_conv);
             ReduceItem& R = c->h_rred.p[item];
@@ -264,15 +260,22 @@ int run_plan(mw_ctx* c, Plan& P)
             (rc = c->h_srows.reserve((size_t)n_slots * NS)))
             return rc;
         const int blocks = n_star_pass > 0 ? (int)((n_star_pass + kBlock - 1) / kBlock) : 0;
+        const int G = P.groups;
+        if (G > 1 && (rc = c->chain.reserve((size_t)(n_slots / G) * std::max<long>(c->n_stars, 1))))
+            return rc;
         std::vector<int> order(n_slots);
         for (int i = 0; i < n_slots; i++) order[i] = i;
-        std::stable_sort(order.begin(), order.end(),
-                         [&](int a, int b) { return P.K[a].bg_mode < P.K[b].bg_mode; });
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            if (a % G != b % G) return a % G < b % G;        // stream groups in order
+            if (P.K[a].bg_mode != P.K[b].bg_mode) return P.K[a].bg_mode < P.K[b].bg_mode;
+            return P.K[a].n_streams < P.K[b].n_streams;
+        });
         for (int oi = 0; oi < n_slots; oi++) {
             const int s = order[oi];
-            const int bg = P.K[s].bg_mode;
-            if (sgroups.empty() || sgroups.back().bg != bg)
-                sgroups.push_back(Group{ bg, oi, 0, total_sblocks, 0 });
+            const int bg = P.K[s].bg_mode, nS = P.K[s].n_streams, cg = s % G;
+            if (sgroups.empty() || sgroups.back().bg != bg || sgroups.back().S != nS ||
+                sgroups.back().chain != cg)
+                sgroups.push_back(Group{ bg, nS, cg, oi, 0, total_sblocks, 0 });
             Group& g = sgroups.back();
             StarLaunch& L = c->h_sitems.p[oi];
             L.star = c->star.p;
@@ -282,8 +285,10 @@ int run_plan(mw_ctx* c, Plan& P)
             L.n_conv = c->n_conv;
             L.block_begin = g.n_blocks;
             L.block_count = blocks;
-            L.out_index = s;
-            L.n_streams = c->S;
+            L.out_index = s / G;          // the vector's per-star row
+            L.n_streams = nS;
+            L.chain = G > 1 ? c->chain.p + (size_t)(s / G) * c->n_stars : nullptr;
+            L.chain_flags = (cg == 0 ? 1 : 0) | (cg == G - 1 ? 2 : 0);
             L.ratio = c->ratio.p;
             L.kw = c->kw.p;
             L.kw3 = c->kw3.p;
@@ -300,7 +305,7 @@ int run_plan(mw_ctx* c, Plan& P)
             total_sblocks += blocks;
         }
         if ((rc = c->spart.reserve((size_t)std::max(total_sblocks, 1) * NS * 2))) return rc;
-        if (P.log10p_host && (rc = c->log10p.reserve((size_t)n_slots * std::max<long>(c->n_stars, 1))))
+        if (P.log10p_host && (rc = c->log10p.reserve((size_t)(n_slots / G) * std::max<long>(c->n_stars, 1))))
             return rc;
     }
 
@@ -336,7 +341,7 @@ int run_plan(mw_ctx* c, Plan& P)
 
     if (timed) MW_HIP_CHECK(hipEventRecord(c->ev[0], st));
     for (const Group& g : igroups) {
-        MW_HIP_CHECK(launch_integral(c->S, g.bg, P.mixed, c->ritems.p + g.first_item, g.n_items,
+        MW_HIP_CHECK(launch_integral(g.S, g.bg, P.mixed, c->ritems.p + g.first_item, g.n_items,
                                      g.n_blocks, c->consts.p, c->ipart.p + (size_t)g.first_block * NI * 2, st));
     }
     if (timed) MW_HIP_CHECK(hipEventRecord(c->ev[1], st));
@@ -348,7 +353,7 @@ int run_plan(mw_ctx* c, Plan& P)
     if (timed) MW_HIP_CHECK(hipEventRecord(c->ev[2], st));
     for (const Group& g : sgroups) {
         if (g.n_blocks == 0) continue;
-        MW_HIP_CHECK(launch_star(c->S, g.bg, P.mixed, c->sitems.p + g.first_item, g.n_items, g.n_blocks,
+        MW_HIP_CHECK(launch_star(g.S, g.bg, P.mixed, c->sitems.p + g.first_item, g.n_items, g.n_blocks,
                                  c->consts.p, c->ratio.p, c->kw.p,
                                  c->spart.p + (size_t)g.first_block * NS * 2,
                                  P.log10p_host ? c->log10p.p : nullptr, c->n_stars, st));
@@ -366,7 +371,7 @@ int run_plan(mw_ctx* c, Plan& P)
     else if (P.stars)
         std::memset(c->h_srows.p, 0, sizeof(double) * n_slots * NS);
     if (P.stars && P.log10p_host && c->n_stars > 0)
-        MW_HIP_CHECK(hipMemcpyAsync(P.log10p_host, c->log10p.p, sizeof(double) * n_slots * c->n_stars,
+        MW_HIP_CHECK(hipMemcpyAsync(P.log10p_host, c->log10p.p, sizeof(double) * (n_slots / P.groups) * c->n_stars,
                                     hipMemcpyDeviceToHost, st));
     MW_HIP_CHECK(hipStreamSynchronize(st));
     if (timed) {
@@ -391,6 +396,16 @@ int make_const(const ASTRONOMY_PARAMETERS& ap, EvalConst* k)
 {
     std::string err;
     if (!make_eval_const(ap, k, &err)) return fail(MW_E_INVALID, err);
+    return MW_OK;
+}
+
+// The constant slots of one model (any number of streams) appended to K.
+int make_consts(const ASTRONOMY_PARAMETERS& ap, std::vector<EvalConst>* K)
+{
+    std::string err;
+    std::vector<EvalConst> k;
+    if (!make_eval_consts(ap, &k, &err)) return fail(MW_E_INVALID, err);
+    K->insert(K->end(), k.begin(), k.end());
     return MW_OK;
 }
 
@@ -490,7 +505,7 @@ int mw_ctx_create(const ASTRONOMY_PARAMETERS* ap, const STAR_POINTS* sp, int fla
     *out = nullptr;
     if (mw_device_count() <= 0) return fail(MW_E_NODEVICE, "no GPU visible");
     if (ap->number_streams < 0 || ap->number_streams > MW_MAX_STREAMS)
-        return fail(MW_E_INVALID, "number_streams must be between 0 and 4");
+        return fail(MW_E_INVALID, "number_streams must be between 0 and MW_MAX_STREAMS (16)");
     if (ap->number_integrals < 1 || !ap->integral) return fail(MW_E_INVALID, "no integration regions");
     std::vector<RegionGeom> geoms;
     for (int g = 0; g < ap->number_integrals; g++) geoms.push_back(geom_of(*ap->integral[g]));
@@ -502,8 +517,8 @@ int mw_ctx_create(const ASTRONOMY_PARAMETERS* ap, const STAR_POINTS* sp, int fla
     c->S = ap->number_streams;
     c->n_params = optimized_count(c->tmpl.ap);
     // Validate the template parameters once.
-    EvalConst k;
-    if ((rc = make_const(c->tmpl.ap, &k))) return rc;
+    std::vector<EvalConst> k;
+    if ((rc = make_consts(c->tmpl.ap, &k))) return rc;
     *out = c.release();
     return MW_OK;
 }
@@ -517,7 +532,7 @@ size_t mw_ctx_resident_bytes(const mw_ctx* c)
     if (!c) return 0;
     size_t b = 0;
     for (const auto& r : c->regions) {
-        b += (r->rq.cap + r->rq32.cap + r->col.cap) * sizeof(double) + r->rstep.cap * sizeof(RStep);
+        b += r->rq.cap * sizeof(double) + r->rstep.cap * sizeof(RStep) + r->col.cap * sizeof(double);
         std::lock_guard<std::mutex> lock(r->level_mu);
         for (const auto& per_level : r->levels)
             for (const auto& q : per_level)
@@ -542,50 +557,66 @@ int mw_ctx_eval(mw_ctx* c, const double* params, int n_vec, int precision, mw_re
     if (precision != MW_PREC_FP64 && precision != MW_PREC_MIXED && precision != MW_PREC_RESAMPLED)
         return fail(MW_E_INVALID, "unknown precision");
     const int G = (int)c->regions.size();
+    const int SG = stream_groups(c->S);
     Plan P;
-    P.K.resize(n_vec);
+    P.groups = SG;
     for (int v = 0; v < n_vec; v++) {
         ModelTemplate m;
         m.load(c->tmpl.ap);
         if (c->n_params > 0) apply_vector(m.ap, params + (size_t)v * c->n_params);
-        int rc = make_const(m.ap, &P.K[v]);
+        int rc = make_consts(m.ap, &P.K);
         if (rc) return rc;
-        for (int g = 0; g < G; g++) {
-            const RegionGeom& rg = c->regions[g]->geom;
-            P.regions.push_back(PlanRegion{ v, g, 0, (long)rg.r_steps * rg.mu_steps * rg.nu_steps });
+        for (int sg = 0; sg < SG; sg++) {
+            for (int g = 0; g < G; g++) {
+                const RegionGeom& rg = c->regions[g]->geom;
+                P.regions.push_back(PlanRegion{ v * SG + sg, g, 0, (long)rg.r_steps * rg.mu_steps * rg.nu_steps });
+            }
         }
     }
     P.integrals = true;
     P.stars = c->n_stars > 0;
     P.mixed = precision != MW_PREC_FP64;
-    P.resample = precision == MW_PREC_RESAMPLED;
+    // a model of more than 4 streams runs on the exact quadrature
+    P.resample = precision == MW_PREC_RESAMPLED && SG == 1;
     P.device_finalize = true;
     P.star_begin = 0;
     P.log10p_host = log10p;
     int rc = run_plan(c, P);
     if (rc) return rc;
     const int NI = integral_comps(), NS = star_comps();
-    c->last_regions.assign(c->h_irows.p, c->h_irows.p + (size_t)n_vec * G * NI);
+    constexpr int RC = 1 + MW_MAX_STREAMS;   // components of a region row of mw_ctx_region_integrals
+    c->last_regions.assign((size_t)n_vec * G * RC, 0.0);
     c->last_n_vec = n_vec;
     const double N = (double)c->n_stars;
     for (int v = 0; v < n_vec; v++) {
+        // region g of stream group sg of vector v: row (v SG + sg) G + g
+        double* lr = c->last_regions.data() + (size_t)v * G * RC;
+        for (int sg = 0; sg < SG; sg++) {
+            for (int g = 0; g < G; g++) {
+                const double* row = c->h_irows.p + ((size_t)(v * SG + sg) * G + g) * NI;
+                if (sg == 0) lr[g * RC] = row[0];
+                const int ns = P.K[v * SG + sg].n_streams;
+                for (int j = 0; j < ns; j++) lr[g * RC + 1 + kMaxStreams * sg + j] = row[1 + j];
+            }
+        }
         mw_result& r = results[v];
         std::memset(&r, 0, sizeof(r));
         r.number_streams = c->S;
-        const double* row = c->h_irows.p + (size_t)v * G * NI;
-        r.background_integral = row[0];
-        for (int s = 0; s < c->S; s++) r.stream_integrals[s] = row[1 + s];
+        r.background_integral = lr[0];
+        for (int s = 0; s < c->S; s++) r.stream_integrals[s] = lr[1 + s];
         for (int g = 1; g < G; g++) {
-            r.background_integral -= row[g * NI];
-            for (int s = 0; s < c->S; s++) r.stream_integrals[s] -= row[g * NI + 1 + s];
+            r.background_integral -= lr[g * RC];
+            for (int s = 0; s < c->S; s++) r.stream_integrals[s] -= lr[g * RC + 1 + s];
         }
         if (P.stars) {
-            const double* sr = c->h_srows.p + (size_t)v * NS;
-            r.num_zero = (int)sr[NS - 1];
-            r.prob_sum = sr[0] - 238.0 * r.num_zero;
+            const double* first = c->h_srows.p + (size_t)(v * SG) * NS;
+            const double* last = c->h_srows.p + (size_t)(v * SG + SG - 1) * NS;
+            r.num_zero = (int)last[NS - 1];
+            r.prob_sum = last[0] - 238.0 * r.num_zero;
             r.likelihood = r.prob_sum / N;
-            r.bg_only_likelihood = sr[1] / N;
-            for (int s = 0; s < c->S; s++) r.stream_only_likelihood[s] = sr[2 + s] / N;
+            r.bg_only_likelihood = first[1] / N;
+            for (int s = 0; s < c->S; s++)
+                r.stream_only_likelihood[s] = c->h_srows.p[(size_t)(v * SG + s / kMaxStreams) * NS + 2 + s % kMaxStreams] / N;
         } else {
             r.likelihood = NAN;
         }
@@ -617,31 +648,34 @@ int mw_hip_calculate_integrals(const ASTRONOMY_PARAMETERS* ap, EVALUATION_STATE*
     int rc = dropin_context(ap, es, sp, &c);
     if (rc) return rc;
     Plan P;
-    P.K.resize(1);
-    if ((rc = make_const(*ap, &P.K[0]))) return rc;
+    if ((rc = make_consts(*ap, &P.K))) return rc;
+    P.groups = (int)P.K.size();
     std::vector<long> ends(es->number_integrals, -1);
-    for (int g = es->current_integral; g < es->number_integrals; g++) {
-        const INTEGRAL_AREA& a = *es->integral[g];
-        const long total = (long)a.r_steps * a.mu_steps * a.nu_steps;
-        long b = a.current_calculation, e = a.max_calculation;
-        if (b < 0) return fail(MW_E_INVALID, "negative current_calculation");
-        if (b >= total) continue;            // nothing left in this region
-        // The reference computes the point at current_calculation before it
-        // tests max_calculation (evaluation_optimized.c:498-522).
-        if (e <= b) e = b + 1;
-        if (e > total) e = total;
-        ends[g] = e;
-        P.regions.push_back(PlanRegion{ 0, g, b, e });
+    for (int sg = 0; sg < P.groups; sg++) {
+        for (int g = es->current_integral; g < es->number_integrals; g++) {
+            const INTEGRAL_AREA& a = *es->integral[g];
+            const long total = (long)a.r_steps * a.mu_steps * a.nu_steps;
+            long b = a.current_calculation, e = a.max_calculation;
+            if (b < 0) return fail(MW_E_INVALID, "negative current_calculation");
+            if (b >= total) continue;            // nothing left in this region
+            // The reference computes the point at current_calculation before it
+            // tests max_calculation (evaluation_optimized.c:498-522).
+            if (e <= b) e = b + 1;
+            if (e > total) e = total;
+            ends[g] = e;
+            P.regions.push_back(PlanRegion{ sg, g, b, e });
+        }
     }
     P.integrals = true;
     P.mixed = dropin_precision_locked() == MW_PREC_MIXED;
     if ((rc = run_plan(c, P))) return rc;
     const int NI = integral_comps();
     for (size_t i = 0; i < P.regions.size(); i++) {
+        const int sg = P.regions[i].slot;
         INTEGRAL_AREA& a = *es->integral[P.regions[i].region];
         const double* row = c->h_irows.p + i * NI;
-        a.background_integral += row[0];
-        for (int s = 0; s < ap->number_streams; s++) a.stream_integrals[s] += row[1 + s];
+        if (sg == 0) a.background_integral += row[0];
+        for (int j = 0; j < P.K[sg].n_streams; j++) a.stream_integrals[kMaxStreams * sg + j] += row[1 + j];
         a.current_calculation = ends[P.regions[i].region];
     }
     es->current_integral = ap->number_integrals;
@@ -663,17 +697,19 @@ int mw_hip_calculate_likelihood(const ASTRONOMY_PARAMETERS* ap, EVALUATION_STATE
     int rc = dropin_context(ap, es, sp, &c);
     if (rc) return rc;
     Plan P;
-    P.K.resize(1);
-    if ((rc = make_const(*ap, &P.K[0]))) return rc;
-    P.K[0].I_bg = es->background_integral;
-    for (int s = 0; s < ap->number_streams; s++) P.K[0].I_st[s] = es->stream_integrals[s];
+    if ((rc = make_consts(*ap, &P.K))) return rc;
+    P.groups = (int)P.K.size();
+    for (int sg = 0; sg < P.groups; sg++) {
+        P.K[sg].I_bg = es->background_integral;
+        for (int j = 0; j < P.K[sg].n_streams; j++) P.K[sg].I_st[j] = es->stream_integrals[kMaxStreams * sg + j];
+    }
     P.stars = true;
     P.mixed = dropin_precision_locked() == MW_PREC_MIXED;
     P.star_begin = std::max(0, es->current_star_point);
     if (P.star_begin > sp->number_stars) P.star_begin = sp->number_stars;
     if ((rc = run_plan(c, P))) return rc;
     const int NS = star_comps();
-    const double* sr = c->h_srows.p;
+    const double* sr = c->h_srows.p + (size_t)(P.groups - 1) * NS;   // the last group finishes the sum
     const int nz = (int)sr[NS - 1];
     es->prob_sum = sr[0] - 238.0 * nz;
     es->num_zero += nz;

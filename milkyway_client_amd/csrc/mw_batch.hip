@@ -131,8 +131,7 @@ int BatchRunner::plan(const std::vector<Slot>& slots, int precision)
             points_ += rd->n_cols * tr.nodes;
             RegionLaunch L;
             std::memset(&L, 0, sizeof(L));
-            L.qw_f32 = (mixed_ && level < 0 && slots[s].k.bg_mode != kBgGeneral) ? 1 : 0;
-            L.rq = L.qw_f32 ? rd->rq32.p : tr.rq;
+            L.rq = tr.rq;
             L.rstep = tr.rstep;
             L.col = rd->col.p;
             L.n_cols = rd->n_cols;
@@ -220,6 +219,7 @@ int BatchRunner::plan(const std::vector<Slot>& slots, int precision)
         Group& g = sgroups_.back();
         StarLaunch L;
         std::memset(&L, 0, sizeof(L));
+        L.chain_flags = 3;                 // batches take models of at most 4 streams
         L.star = c->star.p;
         L.n_stars = c->n_stars;
         L.star_begin = 0;

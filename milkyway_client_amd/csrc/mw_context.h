@@ -281,9 +281,6 @@ struct RegionDev {
     long n_cols = 0;
     int blocks = 0;
     DevBuf<double> rq, col;
-    // rq with FP32 weights for the mixed exact kernels: per point the FP64
-    // distance, then qw rounded to FP32 in the low half of the second word
-    DevBuf<double> rq32;
     DevBuf<RStep> rstep;
     // Host copies of the exact r-step tables, and what the choice of a
     // resampled grid needs: the largest convolution distance (kpc), the
@@ -416,6 +413,7 @@ struct mw_ctx {
     mw::DevBuf<mw::StarLaunch> sitems;
     mw::DevBuf<mw::ReduceItem> rred, sred;
     mw::DevBuf<double> ipart, spart, irows, srows, log10p;
+    mw::DevBuf<double> chain;           // per-star mixture sums between stream groups
     mw::DevBuf<int> first, count;
     mw::DevBuf<int> sep_assign;         // stream separation outputs
     mw::DevBuf<double> sep_xyz, sep_prob;
@@ -429,7 +427,7 @@ struct mw_ctx {
     hipEvent_t ev[6] = {};
     bool have_events = false;
     mw_timing timing{};
-    std::vector<double> last_regions;   // [n_vec][n_regions][kIntComps]
+    std::vector<double> last_regions;   // [n_vec][n_regions][1 + MW_MAX_STREAMS]
     int last_n_vec = 0;
 
     // drop-in cache key

@@ -96,9 +96,6 @@ struct RegionLaunch {
     const double* st_rq[kMaxStreams];
     double st_step[kMaxStreams], st_lo[kMaxStreams];
     int st_rows[kMaxStreams];
-    // 1: rq is the exact tables' copy with FP32 weights (16 bytes per point:
-    // the FP64 distance, then qw in FP32), read by the mixed kernels' FP32 rows
-    int qw_f32;
 };
 
 // Stars resident on the device, SoA.
@@ -120,6 +117,13 @@ struct StarLaunch {
     int bg_count;
     const double* kw3;         // [n_conv] kw_i ratio_i^3 (the mixed pass's stream weights / r_star^3)
     double coarse_pr;          // mixed pass: p r_star above which a stream's exponents are split exactly
+    // Models of more than 4 streams run one star pass per group of 4, in
+    // order; each star's mixture sum is carried from pass to pass in
+    // chain[star] (bit 0 of chain_flags: the first group, which starts the
+    // sum with the background; bit 1: the last, which finishes it).  A model
+    // of at most 4 streams has chain_flags 3 and no chain.
+    double* chain;
+    int chain_flags;
 };
 
 // Constants of the stream separation pass (separation.c:37-254).
@@ -147,6 +151,13 @@ void build_star_soa(const STAR_POINTS& sp, std::vector<double>* out);
 // Fills an EvalConst from the model parameters held in ap; returns false
 // (with a message) if the parameters cannot be evaluated here.
 bool make_eval_const(const ASTRONOMY_PARAMETERS& ap, EvalConst* k, std::string* err);
+// Constant slots of a model with any number of streams: stream_groups(S)
+// slots, slot g holding streams [4 g, 4 g + 4) with the background (slot 0
+// only; later slots have bg_mode kBgZeroQ, which skips the background's
+// work) and the mixture's normalisation over every stream.  One slot, equal
+// to make_eval_const's, for S <= 4.
+inline int stream_groups(int S) { return S <= kMaxStreams ? 1 : (S + kMaxStreams - 1) / kMaxStreams; }
+bool make_eval_consts(const ASTRONOMY_PARAMETERS& ap, std::vector<EvalConst>* out, std::string* err);
 
 // The stream separation's output frame: the wedge's normal (stripe_normal /
 // sgr_stripe_normal) and the rotation taking it to +z (get_transform).

@@ -332,20 +332,19 @@ __device__ __forceinline__ void bg_point(double r, double qw, const Quad& q, con
     }
 }
 
-// The Hernquist background at one point in FP32, for the mixed exact
-// quadrature: qw / (rg (rg + r0)^3) with rg^2 = t^2 + e, t = p r - m taken in
-// FP64 (the shifted quadratic loses nothing to cancellation) and rounded
+// The Hernquist background's density at one point in FP32, for the mixed
+// integral pass: 1 / (rg (rg + r0)^3) with rg^2 = t^2 + e, t = p r - m taken
+// in FP64 (the shifted quadratic loses nothing to cancellation) and rounded
 // once.  1/rg is v_rsq_f32; 1/(rg + r0) is v_rcp_f32 refined by one FP32
-// Newton step.  Measured on gfx950 over whole binades, v_rsq_f32 errs by
-// at most 0.87 ulp with a mean of -0.08 ulp and v_rcp_f32 by 0.88 ulp with a
+// Newton step.  Measured on gfx950 over whole binades, v_rsq_f32 errs by at
+// most 0.87 ulp with a mean of -0.08 ulp and v_rcp_f32 by 0.88 ulp with a
 // mean of -0.07 ulp, which the Newton step takes to 0.5 ulp and a mean of
-// 0.0001 ulp; v_sqrt_f32 (mean -0.15 ulp, amplified up to four times by
-// the cube) is not used.  What remains systematic is the rsq's bias, which
-// enters the term as b (3 rg / (rg + r0) - 1), |b| <= 0.08 ulp: below 2e-8
-// relative in the background integral.  The term is added to a per-row
-// FP32 sum (at most 1024 positive terms, rounded to nearest).
-__device__ __forceinline__ void bg_point_f32(double r, float qw, const Quad& q, float r0, float e,
-                                             float& bg)
+// 0.0001 ulp; v_sqrt_f32 (mean -0.15 ulp, amplified up to four times by the
+// cube) is not used.  What stays systematic is the rsq's bias, which enters
+// the density as b (1 - 3 rg / (rg + r0)), |b| <= 0.08 ulp: at most 2e-8
+// relative.  The caller weights it and accumulates in FP64: summing such
+// terms in FP32 loses the small ones one-sidedly.
+__device__ __forceinline__ float hernquist_f32(double r, const Quad& q, float r0, float e)
 {
     const float t = (float)fma(q.p, r, -q.m);
     const float rg2 = fmaf(t, t, e);
@@ -353,44 +352,20 @@ __device__ __forceinline__ void bg_point_f32(double r, float qw, const Quad& q, 
     const float s = fmaf(rg2, y, r0);
     float z = __builtin_amdgcn_rcpf(s);
     z = fmaf(z, fmaf(-s, z, 1.0f), z);
-    bg = fmaf(qw * y, z * z * z, bg);
+    return y * (z * z * z);
 }
 
-// One r-step row of the mixed exact quadrature: the background (Hernquist
-// profile) by bg_point_f32 and each stream's 2^-(t^2) by v_exp_f32, t in
-// FP64, every term weighted by the FP32 qw and summed per row in FP32; the
-// row sums are returned in FP64, where the caller scales them and adds them
-// to the column's Kahan sums.  Row entries are 16 bytes: the FP64 distance,
-// then qw as an FP32 value in the low half of the second word.
-template <int S, int BG, int UNROLL>
-__device__ __forceinline__ void mixed_row_f32(const double2* row, int n, const Sight<S>& v, float r0,
-                                              float e, double& bg, double (&st)[kMaxStreams])
-{
-    float b = 0.0f;
-    float a[S > 0 ? S : 1];
-#pragma unroll
-    for (int s = 0; s < (S > 0 ? S : 1); s++) a[s] = 0.0f;
-#pragma unroll UNROLL
-    for (int i = 0; i < n; i++) {
-        const double2 en = row[i];
-        const float qw = __uint_as_float((unsigned)__double_as_longlong(en.y));
-        if constexpr (BG != kBgZeroQ) bg_point_f32(en.x, qw, v.bg, r0, e, b);
-#pragma unroll
-        for (int s = 0; s < S; s++) {
-            const double t = fma(v.st[s].p, en.x, -v.st[s].m);
-            a[s] = fmaf(qw, __builtin_amdgcn_exp2f(-(float)(t * t)), a[s]);
-        }
-    }
-    bg = (double)b;
-#pragma unroll
-    for (int s = 0; s < S; s++) st[s] = (double)a[s];
-}
-
+// r0f and ebf: the background's r0 and the sight's e in FP32 (mixed
+// integral pass, Hernquist profile, whose density is taken in FP32).
 template <int S, int BG, bool MIXED, bool SHIFTED = false>
 __device__ __forceinline__ void point_terms(double r, double qw, const Sight<S>& v,
-                                            const EvalConst& K, double& bg, double (&st)[kMaxStreams])
+                                            const EvalConst& K, double& bg, double (&st)[kMaxStreams],
+                                            float r0f = 0.0f, float ebf = 0.0f)
 {
-    bg_point<BG>(r, qw, v.bg, K, bg);
+    if constexpr (MIXED && SHIFTED && (BG == kBgHernquist || BG == kBgHernquistAux))
+        bg = fma(qw, (double)hernquist_f32(r, v.bg, r0f, ebf), bg);
+    else
+        bg_point<BG>(r, qw, v.bg, K, bg);
 #pragma unroll
     for (int s = 0; s < S; s++) {
         if constexpr (MIXED && SHIFTED) {
@@ -728,9 +703,6 @@ __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
         double acc[1 + S], cmp[1 + S];
 #pragma unroll
         for (int c = 0; c <= S; c++) acc[c] = cmp[c] = 0.0;
-        // FP32 rows (mixed, exact tables, Hernquist profile): the table's
-        // weights are FP32 and the per-row sums run in FP32.
-        const bool f32_rows = MIXED && BG != kBgGeneral && it.qw_f32;
         const float r0f = (float)K.r0, ebf = (float)sight.bg.e;
 
         const int rt = kTileD2 / n;   // r-steps per LDS tile (host guarantees n <= kTileD2)
@@ -745,22 +717,10 @@ __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
                 double bg = 0.0;
                 double st[kMaxStreams] = { 0.0, 0.0, 0.0, 0.0 };
                 const double2* row = tile + kk * n;
-                if constexpr (MIXED && BG != kBgGeneral) {
-                    if (f32_rows) {
-                        mixed_row_f32<S, BG, UNROLL>(row, n, sight, r0f, ebf, bg, st);
-                    } else {
 #pragma unroll UNROLL
-                        for (int i = 0; i < n; i++) {
-                            const double2 p = row[i];
-                            point_terms<S, BG, MIXED, MIXED>(p.x, p.y, sight, K, bg, st);
-                        }
-                    }
-                } else {
-#pragma unroll UNROLL
-                    for (int i = 0; i < n; i++) {
-                        const double2 p = row[i];
-                        point_terms<S, BG, MIXED, MIXED>(p.x, p.y, sight, K, bg, st);
-                    }
+                for (int i = 0; i < n; i++) {
+                    const double2 p = row[i];
+                    point_terms<S, BG, MIXED, MIXED>(p.x, p.y, sight, K, bg, st, r0f, ebf);
                 }
                 const RStep& rs = it.rstep[k];
                 double f = rs.reff_xr_rp3 * rs.irv;
@@ -1046,32 +1006,45 @@ __device__ __forceinline__ void star_block(const StarLaunch& it, long lb,
     star_densities<S, BG, MIXED>(it.star, N, i, it.n_conv, K, conv_ratio, conv_kw, reff, bg, st, it.kw3,
                                  it.coarse_pr, it.bg_rq, it.bg_count);
 
-    // Mixture (evaluation_optimized.c:674-716).
+    // Mixture (evaluation_optimized.c:674-716).  A model of more than 4
+    // streams is summed over passes of 4 streams in stream order: the first
+    // starts with the background term, each later pass continues the sum
+    // from it.chain, and the last divides and takes the logarithm.
     DD out[kStarComps];
 #pragma unroll
     for (int c = 0; c < kStarComps; c++) out[c] = DD{ 0.0, 0.0 };
-    double bg_only = (bg / K.I_bg) * K.exp_bg_w;
-    double p = bg_only;
-    out[1].hi = (bg_only == 0.0) ? -238.0 : log10(bg_only / K.sum_exp_w);
+    const bool first = (it.chain_flags & 1) != 0, last = (it.chain_flags & 2) != 0;
+    double p;
+    if (first) {
+        const double bg_only = (bg / K.I_bg) * K.exp_bg_w;
+        p = bg_only;
+        out[1].hi = (bg_only == 0.0) ? -238.0 : log10(bg_only / K.sum_exp_w);
+    } else {
+        p = it.chain[i];
+    }
 #pragma unroll
     for (int s = 0; s < S; s++) {
         const double so = (st[s] * reff * K.on[s]) / K.I_st[s] * K.exp_st_w[s];
         p += so;
         out[2 + s].hi = (so == 0.0) ? -238.0 : log10(so / K.sum_exp_w);
     }
-    p /= K.sum_exp_w;
-    double lp;
-    if (p != 0.0) {
-        lp = log10(p);
-        out[0].hi = lp;
-    } else {
-        lp = -238.0;
-        out[kStarComps - 1].hi = 1.0;
+    double lp = 0.0;
+    if (last) {
+        p /= K.sum_exp_w;
+        if (p != 0.0) {
+            lp = log10(p);
+            out[0].hi = lp;
+        } else {
+            lp = -238.0;
+            out[kStarComps - 1].hi = 1.0;
+        }
+    } else if (valid) {
+        it.chain[idx] = p;
     }
     if (!valid) {
 #pragma unroll
         for (int c = 0; c < kStarComps; c++) out[c] = DD{ 0.0, 0.0 };
-    } else if (log10p) {
+    } else if (log10p && last) {
         log10p[(long)it.out_index * log10p_stride + idx] = lp;
     }
     if constexpr (MIXED) {

@@ -101,6 +101,7 @@ int mw_ctx_separation(mw_ctx* c, const double* params, int precision, const doub
             return rc;
         StarLaunch it;
         std::memset(&it, 0, sizeof(it));
+        it.chain_flags = 3;
         it.star = c->star.p;
         it.n_stars = N;
         it.n_conv = c->n_conv;

@@ -246,6 +246,8 @@ int mw_server_add_wedge(mw_server* s, const ASTRONOMY_PARAMETERS* ap, const STAR
                         int* wedge_id)
 {
     if (!s || !ap || !wedge_id) return fail(MW_E_INVALID, "null argument");
+    if (ap->number_streams > kMaxStreams)
+        return fail(MW_E_INVALID, "the request server takes models of at most 4 streams");
     mw_ctx* c = nullptr;
     {
         std::lock_guard<std::mutex> lk(s->mu);

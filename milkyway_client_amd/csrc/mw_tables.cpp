@@ -653,7 +653,8 @@ bool make_eval_const(const ASTRONOMY_PARAMETERS& ap, EvalConst* k, std::string* 
     std::memset(k, 0, sizeof(*k));
     const int S = ap.number_streams;
     if (S < 0 || S > kMaxStreams) {
-        *err = "number_streams must be between 0 and 4";
+        *err = "one constant slot holds at most 4 streams (batches and the request server take at most "
+               "4 per model; mw_ctx_eval and the drop-in entry points take more, in groups of 4)";
         return false;
     }
     if (ap.aux_bg_profile != 0 && ap.aux_bg_profile != 1) {
@@ -716,6 +717,40 @@ bool make_eval_const(const ASTRONOMY_PARAMETERS& ap, EvalConst* k, std::string* 
         sum += std::exp(ap.stream_weights[s]);
     }
     k->sum_exp_w = sum * 0.001;
+    return true;
+}
+
+bool make_eval_consts(const ASTRONOMY_PARAMETERS& ap, std::vector<EvalConst>* out, std::string* err)
+{
+    const int S = ap.number_streams;
+    if (S < 0) {
+        *err = "negative number_streams";
+        return false;
+    }
+    const int G = stream_groups(S);
+    out->assign(G, EvalConst{});
+    for (int g = 0; g < G; g++) {
+        // the streams [kMaxStreams g, kMaxStreams g + n) as a model of their own
+        ASTRONOMY_PARAMETERS sub = ap;
+        const int first = kMaxStreams * g;
+        sub.number_streams = std::min(kMaxStreams, S - first);
+        if (S > 0) {
+            sub.stream_weights = ap.stream_weights + first;
+            sub.stream_parameters = ap.stream_parameters + first;
+        }
+        if (!make_eval_const(sub, &(*out)[g], err)) return false;
+    }
+    if (G > 1) {
+        // the mixture's normalisation over every stream, summed in stream
+        // order as a single group would (evaluation_optimized.c:639-646)
+        double sum = std::exp(ap.background_weight);
+        for (int s = 0; s < S; s++) sum += std::exp(ap.stream_weights[s]);
+        for (int g = 0; g < G; g++) {
+            (*out)[g].sum_exp_w = sum * 0.001;
+            // groups after the first carry no background: its work is skipped
+            if (g > 0) (*out)[g].bg_mode = kBgZeroQ;
+        }
+    }
     return true;
 }
 

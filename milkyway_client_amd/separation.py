@@ -29,7 +29,7 @@ CLIENT_PATH = LIB_PATH.parent / "milkyway_separation_hip"
 
 MW_OK, MW_E_INVALID, MW_E_HIP, MW_E_NOMEM, MW_E_NODEVICE = 0, 1, 2, 3, 4
 PREC_FP64, PREC_MIXED, PREC_RESAMPLED = 0, 1, 2
-MAX_STREAMS = 4
+MAX_STREAMS = 16   # MW_MAX_STREAMS: streams per model in the context API
 CTX_TIMING = 1
 
 # Every symbol the C ABI exports (include/mw_hip.h).
