@@ -335,20 +335,27 @@ def full_size_fp64(steps: int, warmup: int, precision=sep.PREC_FP64):
     }
 
 
+SERVING_SET = 400
+
+
 def serving_run(n_requests: int, rates, variant: int, bpc: int):
     """Poisson arrival streams at each rate over 8 resident wedges; requests
-    are resampled mixed precision with a 10% FP64 share.  The plain
-    one-at-a-time FP64 server is run at the lowest rate only (it saturates
-    at about 10 requests/s)."""
+    are resampled mixed precision with a 10% FP64 share.  Every rate serves
+    the same request sequence (its first max(n_requests, min(400, 2 rate))
+    requests), so that only the arrival rate changes between rows.  The
+    plain one-at-a-time FP64 server is run at the lowest rate only (it
+    saturates at about 10 requests/s)."""
     wedges = serving.make_wedges(8, seed=4000)
     holders = [(ParamsHolder(p), StarsHolder(s)) for p, s, _ in wedges]
     out = {"wedges": "8 resident wedges from the batch shapes, 1-4 streams, 10% FP64 requests",
+           "requests": "one seeded sequence of 400 requests; each rate serves its first n",
            "sweep": []}
+    all_reqs = serving.make_requests(wedges, SERVING_SET, seed=5000, fast_precision=sep.PREC_RESAMPLED)
     with sep.Server(max_inflight=3, max_batch=32, variant=variant, blocks_per_cu=bpc) as srv:
         ids = [srv.add_wedge(ph, sh) for ph, sh in holders]
         for k, rate in enumerate(rates):
-            n = max(n_requests, min(400, int(2 * rate)))
-            reqs = serving.make_requests(wedges, n, seed=5000 + k, fast_precision=sep.PREC_RESAMPLED)
+            n = max(n_requests, min(SERVING_SET, int(2 * rate)))
+            reqs = all_reqs[:n]
             arr = serving.poisson_arrivals(len(reqs), rate, seed=6000 + k)
             b0 = srv.stats().batches
             v = serving.run(srv, ids, reqs, arr)
@@ -368,7 +375,7 @@ def serving_run(n_requests: int, rates, variant: int, bpc: int):
             out["sweep"].append(row)
         out["largest_batch"] = int(srv.stats().largest_batch)
     rate = rates[0]
-    reqs = serving.make_requests(wedges, n_requests, seed=5000, fast_precision=sep.PREC_RESAMPLED)
+    reqs = all_reqs[:n_requests]
     arr = serving.poisson_arrivals(len(reqs), rate, seed=6000)
     with sep.Server(max_inflight=1, max_batch=1) as srv:
         ids = [srv.add_wedge(ph, sh) for ph, sh in holders]

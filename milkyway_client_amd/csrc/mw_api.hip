@@ -293,7 +293,7 @@ int run_plan(mw_ctx* c, Plan& P)
             L.kw = c->kw.p;
             L.kw3 = c->kw3.p;
             L.coarse_pr = c->star_coarse_pr;
-            const int sl = (P.resample && P.mixed && bg == kBgHernquist) ? star_bg_level(*c, P.K[s].q) : -1;
+            const int sl = (P.resample && P.mixed && bg == kBgHernquist) ? star_bg_level(*c, P.K[s].q, P.K[s].r0) : -1;
             if ((rc = star_bg_table(*c, sl, &L.bg_rq, &L.bg_count))) return rc;
             ReduceItem& R = c->h_sred.p[oi];
             R.block_begin = total_sblocks;

@@ -49,7 +49,7 @@ int BatchRunner::table_choice(const RegionDev& rd, const EvalConst& k) const
 // batches of the Hernquist profile), or -1 for the exact convolution.
 int BatchRunner::star_level(const mw_ctx& c, const EvalConst& k) const
 {
-    return (resample_ && mixed_ && k.bg_mode == kBgHernquist) ? star_bg_level(c, k.q) : -1;
+    return (resample_ && mixed_ && k.bg_mode == kBgHernquist) ? star_bg_level(c, k.q, k.r0) : -1;
 }
 
 bool BatchRunner::same_modes(const std::vector<Slot>& slots) const

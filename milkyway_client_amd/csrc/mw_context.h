@@ -347,9 +347,20 @@ double bg_theta(const RegionDev& rd, double q);
 // 0.2 x 2.17 x theta and 0.3 mag), or -1 if none is fine enough.
 int bg_grid_level(const RegionDev& rd, double q);
 
+// The parameter range over which the resampled quadrature's folds were
+// measured against the exact point sets (the reference's own bounds,
+// background_min / background_max and stream_min / stream_max of its
+// parameter files: q 0.3-1.0, r0 1-30 kpc, sigma 0.1-20 kpc; q up to 1.6 on
+// the GPU sweep).  Outside it a region keeps the exact tables and a star
+// keeps its exact convolution; wider streams only make the folds more
+// accurate, so sigma has no upper limit.
+constexpr double kFoldQMin = 0.3, kFoldQMax = 1.6, kFoldR0Min = 1.0, kFoldR0Max = 30.0;
+constexpr double kFoldSigmaMin = 0.1;
+bool in_measured_domain(double q, double r0);
+
 // Resampled table index to use for region rd under the model k, or -1 for
-// the exact tables (resampling off, a background mode it does not cover, or
-// no gain).  Pure: builds nothing.
+// the exact tables (resampling off, a background mode it does not cover, a
+// model outside the measured domain, or no gain).  Pure: builds nothing.
 int select_level(const RegionDev& rd, const EvalConst& k, bool resample);
 
 // Node spacing for the kernels' stream recurrence on table `level`: the
@@ -467,10 +478,11 @@ namespace mw {
 // context's stars.
 // Tabulates mw_ctx::star_sin2_q from the star list (SoA, n stars).
 void tabulate_star_theta(mw_ctx& c, const double* soa, long n);
-// Background table index for the star pass under flattening q, or -1 when
-// the exact convolution is used (no stars, q outside the table, no level
-// fine enough, or no fewer nodes than the convolution's points).  Pure.
-int star_bg_level(const mw_ctx& c, double q);
+// Background table index for the star pass under flattening q and scale r0,
+// or -1 when the exact convolution is used (no stars, q outside the table
+// or the measured domain, no level fine enough, or no fewer nodes than the
+// convolution's points).  Pure.
+int star_bg_level(const mw_ctx& c, double q, double r0);
 // That table (built on first use): rq (r / r_star, weight / r_star^3) pairs
 // and their number; nullptr and 0 for level -1.
 int star_bg_table(mw_ctx& c, int level, const double** rq, int* count);

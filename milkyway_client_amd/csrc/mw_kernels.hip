@@ -332,40 +332,11 @@ __device__ __forceinline__ void bg_point(double r, double qw, const Quad& q, con
     }
 }
 
-// The Hernquist background's density at one point in FP32, for the mixed
-// integral pass: 1 / (rg (rg + r0)^3) with rg^2 = t^2 + e, t = p r - m taken
-// in FP64 (the shifted quadratic loses nothing to cancellation) and rounded
-// once.  1/rg is v_rsq_f32; 1/(rg + r0) is v_rcp_f32 refined by one FP32
-// Newton step.  Measured on gfx950 over whole binades, v_rsq_f32 errs by at
-// most 0.87 ulp with a mean of -0.08 ulp and v_rcp_f32 by 0.88 ulp with a
-// mean of -0.07 ulp, which the Newton step takes to 0.5 ulp and a mean of
-// 0.0001 ulp; v_sqrt_f32 (mean -0.15 ulp, amplified up to four times by the
-// cube) is not used.  What stays systematic is the rsq's bias, which enters
-// the density as b (1 - 3 rg / (rg + r0)), |b| <= 0.08 ulp: at most 2e-8
-// relative.  The caller weights it and accumulates in FP64: summing such
-// terms in FP32 loses the small ones one-sidedly.
-__device__ __forceinline__ float hernquist_f32(double r, const Quad& q, float r0, float e)
-{
-    const float t = (float)fma(q.p, r, -q.m);
-    const float rg2 = fmaf(t, t, e);
-    const float y = __builtin_amdgcn_rsqf(rg2);
-    const float s = fmaf(rg2, y, r0);
-    float z = __builtin_amdgcn_rcpf(s);
-    z = fmaf(z, fmaf(-s, z, 1.0f), z);
-    return y * (z * z * z);
-}
-
-// r0f and ebf: the background's r0 and the sight's e in FP32 (mixed
-// integral pass, Hernquist profile, whose density is taken in FP32).
 template <int S, int BG, bool MIXED, bool SHIFTED = false>
 __device__ __forceinline__ void point_terms(double r, double qw, const Sight<S>& v,
-                                            const EvalConst& K, double& bg, double (&st)[kMaxStreams],
-                                            float r0f = 0.0f, float ebf = 0.0f)
+                                            const EvalConst& K, double& bg, double (&st)[kMaxStreams])
 {
-    if constexpr (MIXED && SHIFTED && (BG == kBgHernquist || BG == kBgHernquistAux))
-        bg = fma(qw, (double)hernquist_f32(r, v.bg, r0f, ebf), bg);
-    else
-        bg_point<BG>(r, qw, v.bg, K, bg);
+    bg_point<BG>(r, qw, v.bg, K, bg);
 #pragma unroll
     for (int s = 0; s < S; s++) {
         if constexpr (MIXED && SHIFTED) {
@@ -703,7 +674,6 @@ __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
         double acc[1 + S], cmp[1 + S];
 #pragma unroll
         for (int c = 0; c <= S; c++) acc[c] = cmp[c] = 0.0;
-        const float r0f = (float)K.r0, ebf = (float)sight.bg.e;
 
         const int rt = kTileD2 / n;   // r-steps per LDS tile (host guarantees n <= kTileD2)
         const double2* rq = reinterpret_cast<const double2*>(it.rq);
@@ -720,7 +690,7 @@ __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
 #pragma unroll UNROLL
                 for (int i = 0; i < n; i++) {
                     const double2 p = row[i];
-                    point_terms<S, BG, MIXED, MIXED>(p.x, p.y, sight, K, bg, st, r0f, ebf);
+                    point_terms<S, BG, MIXED, MIXED>(p.x, p.y, sight, K, bg, st);
                 }
                 const RStep& rs = it.rstep[k];
                 double f = rs.reff_xr_rp3 * rs.irv;

@@ -164,15 +164,22 @@ int bg_grid_level(const RegionDev& rd, double q)
     return t < kBgLevels ? t : -1;
 }
 
+bool in_measured_domain(double q, double r0)
+{
+    return q >= kFoldQMin && q <= kFoldQMax && r0 >= kFoldR0Min && r0 <= kFoldR0Max;
+}
+
 int select_level(const RegionDev& rd, const EvalConst& k, bool resample)
 {
     if (!resample || rd.h_rq.empty()) return -1;
     if (k.bg_mode != kBgHernquist && k.bg_mode != kBgHernquistAux) return -1;
+    if (!in_measured_domain(k.q, k.r0)) return -1;
     double sigma = INFINITY;
     for (int s = 0; s < k.n_streams; s++) {
         if (k.on[s] == 0.0 || !(k.inv_2s2[s] > 0.0)) continue;
         sigma = std::min(sigma, std::sqrt(0.5 / k.inv_2s2[s]));
     }
+    if (sigma < kFoldSigmaMin) return -1;
     const double b = bg_width(rd, k.q);
     // grid in magnitude
     const double dg = std::min(0.3 * kMagPerLn * sigma / rd.r_max, 0.2 * kMagPerLn * b / kSunR0);
@@ -425,10 +432,11 @@ double star_bg_delta(int level) { return kDeltaBgMag * std::pow(2.0, -0.5 * leve
 
 }  // namespace
 
-int star_bg_level(const mw_ctx& c, double q)
+int star_bg_level(const mw_ctx& c, double q, double r0)
 {
     if (c.n_stars < 1 || c.conv_dx.size() < 2 || c.star_sin2_q.size() != (size_t)kWidthQ || !(q > 0.0))
         return -1;
+    if (!in_measured_domain(q, r0)) return -1;
     // sin^2 theta falls as q grows, so the entry at the tabulated q just
     // above q bounds it from below.
     const double x = (std::log2(q) + kWidthQ0) * 4.0;
@@ -441,7 +449,8 @@ int star_bg_level(const mw_ctx& c, double q)
     // the convolution tables (n = 60, 120) for every (rg_min, P, r0, r_star)
     // from (0.3-8 kpc, 0.5-11, 1-14 kpc, 1-60 kpc), towards and away from
     // the centre: relative error of a star's background term below 2.5e-10
-    // (7.8e-9 at 0.15, 2.4e-7 at the integral's 0.2).
+    // (7.8e-9 at 0.15, 2.4e-7 at the integral's 0.2); the same holds for r0
+    // up to 30 kpc.
     const double dg = std::min(0.1 * kMagPerLn * theta, kDeltaBgMag);
     if (!(dg > 0.0)) return -1;
     const int level = level_for(dg, kDeltaBgMag);

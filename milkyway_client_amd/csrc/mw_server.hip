@@ -9,10 +9,16 @@
 // batches run at once on separate streams, so requests that arrive while
 // others are on the GPU are admitted as runners and memory free up, and
 // their work overlaps on the device.  FP64 batches, far slower per
-// evaluation, hold at most half the runners and run their persistent
-// grid at no more than 2 workgroups per CU, so faster batches find both a
-// runner and free wave slots.  Each request's result is bitwise the same
-// alone and in any mix (per-request fixed-order reductions).
+// evaluation, are bounded by their estimated work (kFp64BatchPoints
+// integrand evaluations, about a tenth of a second, and at least one
+// request), so no batch holds the device for long.  An FP64 batch runs its
+// persistent grid at no more than 2 workgroups per CU, which leaves wave
+// slots on every CU to concurrent fast batches (the VALU-bound integral
+// kernel loses about 5% at 2 workgroups per CU against 5).  While fast
+// requests are waiting or running, FP64 batches hold at most half the
+// runners; with none, they take every runner, so that an FP64 backlog under
+// overload keeps the device full.  Each request's result is bitwise the
+// same alone and in any mix (per-request fixed-order reductions).
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -30,6 +36,10 @@ using namespace mw;
 namespace {
 
 using Clock = std::chrono::steady_clock;
+
+// Integrand evaluations per FP64 batch (cells x convolution points summed
+// over its requests): about 0.1 s of the FP64 kernel at 3 streams.
+constexpr double kFp64BatchPoints = 4e10;
 
 double now_s()
 {
@@ -92,9 +102,15 @@ bool mw_server::launch_locked(InFlight& f, std::unique_lock<std::mutex>& lk)
     // in arrival order, up to max_batch vectors, while the batch's scratch
     // fits the budget; requests of other classes keep their places.
     int fp64_busy = 0;
-    for (const InFlight& g : slots)
-        if (g.busy && !g.reqs.empty() && g.reqs.front().precision == MW_PREC_FP64) fp64_busy++;
-    const bool allow_fp64 = cfg.max_inflight == 1 || fp64_busy < std::max(1, cfg.max_inflight / 2);
+    bool fast_running = false, fast_waiting = false;
+    for (const InFlight& g : slots) {
+        if (!g.busy || g.reqs.empty()) continue;
+        if (g.reqs.front().precision == MW_PREC_FP64) fp64_busy++;
+        else fast_running = true;
+    }
+    for (const Request& r : pending) fast_waiting |= r.precision != MW_PREC_FP64;
+    const bool fast = fast_running || fast_waiting;
+    const bool allow_fp64 = cfg.max_inflight == 1 || !fast || fp64_busy < std::max(1, cfg.max_inflight / 2);
     int cls = -1;
     for (const Request& r : pending) {
         if (r.precision == MW_PREC_FP64 && !allow_fp64) continue;
@@ -108,10 +124,22 @@ bool mw_server::launch_locked(InFlight& f, std::unique_lock<std::mutex>& lk)
     size_t scratch = 0;
     const size_t budget = cfg.hbm_budget_bytes;
     bool full = false;
+    double points = 0.0;
     for (const Request& r : pending) {
         if (full || r.precision != cls || (int)take.size() >= cfg.max_batch_vectors) {
             rest.push_back(r);
             continue;
+        }
+        if (cls == MW_PREC_FP64) {
+            double pts = 0.0;
+            for (const auto& rd : wedges[r.wedge]->regions)
+                pts += (double)rd->geom.r_steps * rd->geom.mu_steps * rd->geom.nu_steps * rd->n_conv;
+            if (!take.empty() && points + pts > kFp64BatchPoints) {
+                full = true;   // the rest of the class waits for the next batch
+                rest.push_back(r);
+                continue;
+            }
+            points += pts;
         }
         std::vector<Slot> trial = slots_v;
         trial.push_back(Slot{ wedges[r.wedge].get(), r.k });
@@ -137,7 +165,7 @@ bool mw_server::launch_locked(InFlight& f, std::unique_lock<std::mutex>& lk)
     const int precision = f.reqs.front().precision;
     lk.unlock();
     // An FP64 batch's persistent grid takes at most 2 workgroups per CU,
-    // leaving wave slots on every CU to a concurrent faster batch.
+    // leaving wave slots on every CU to fast batches.
     const int bpc = cfg.blocks_per_cu > 0 ? cfg.blocks_per_cu : 2;
     f.runner->set_variant(cfg.variant, precision == MW_PREC_FP64 ? std::min(bpc, 2) : bpc);
     int rc = hipSetDevice(device) == hipSuccess ? MW_OK : MW_E_HIP;

@@ -80,14 +80,50 @@ def test_disabled_stream_under_the_recurrence():
     assert e["I_bg"] <= INT_REL_TOL and e["I_st"] <= INT_REL_TOL
 
 
-def test_narrowest_streams_far_out():
-    """sigma = 1 kpc (the parameter files' lower bound) for every stream."""
+@pytest.mark.parametrize("sigma", [1.0, 0.3, 0.1])
+def test_narrowest_streams_far_out(sigma):
+    """Narrow streams for every stream, down to sigma = 0.1 kpc (the
+    smallest stream_min of the reference's parameter files,
+    tests/astronomy_parameters-11-small.txt; the synthetic workloads draw
+    1-5 kpc)."""
     p, stars = synth.workload(97, LONG_R, 3)
     for s in p.streams:
-        s.parameters[4] = 1.0
+        s.parameters[4] = sigma
     e, grew = errors_vs_oracle(p, stars)
     assert grew
     assert e["lnP"] <= LNP_TOL and e["I_st"] <= INT_REL_TOL and e["I_bg"] <= INT_REL_TOL
+
+
+@pytest.mark.parametrize("field,value", [("sigma", 0.05), ("r0", 0.5), ("r0", 40.0), ("q", 0.2),
+                                         ("q", 2.0)])
+def test_outside_the_measured_domain_keeps_the_exact_tables(field, value):
+    """Models outside the range the folds were measured over (sigma below
+    0.1 kpc, r0 outside 1-30 kpc, q outside 0.3-1.6) run the exact tables
+    in every region and the exact convolution in the star pass: the
+    resampled request then gives the mixed path's result bitwise (for a
+    narrow stream, whose width does not enter the star pass's background
+    fold, the integrals bitwise and every star within the bar)."""
+    p, stars = synth.workload(98, synth.WedgeShape(70, 24, 16, 30, 500), 2)
+    if field == "sigma":
+        p.streams[0].parameters[4] = value
+    elif field == "r0":
+        p.background_parameters[2] = value
+    else:
+        p.background_parameters[1] = value
+    ph, sh = ParamsHolder(p), StarsHolder(stars)
+    with sep.Context(ph, sh, timing=True) as ctx:
+        rs = ctx.eval(p.parameter_vector(), precision=sep.PREC_RESAMPLED, per_star=True)[0]
+        tm = ctx.timing()
+        mx = ctx.eval(p.parameter_vector(), precision=sep.PREC_MIXED, per_star=True)[0]
+    assert tm.integral_points == sum(g.cells for g in p.regions) * p.convolve
+    assert tm.integral_points_recurrence == 0
+    assert rs.background_integral == mx.background_integral
+    assert np.array_equal(rs.stream_integrals, mx.stream_integrals)
+    if field == "sigma":
+        assert math.log(10) * np.max(np.abs(rs.log10_p - mx.log10_p)) <= LNP_TOL
+    else:
+        assert rs.likelihood == mx.likelihood
+        assert np.array_equal(rs.log10_p, mx.log10_p)
 
 
 def test_sight_through_the_galactic_centre():
@@ -292,21 +328,22 @@ def test_aux_profile_fixture():
 
 
 def sweep_vectors(p, count, seed):
-    """Seeded vectors over the synthetic ranges, and beyond them at the
-    extremes: q from 0.3 to 1.0 and one above 1, r0 at 2 and 12 kpc, sigma
-    from 0.6 to 12 kpc, stream centres from 2 to 60 kpc, and streams switched
-    off (|sigma| < 1e-4)."""
+    """Seeded vectors over the reference's parameter range
+    (background_min / background_max and stream_min / stream_max of its
+    parameter files: q 0.3-1.0, r0 1-30 kpc, stream centres 0.7-57.5 kpc,
+    sigma 0.1-20 kpc), q beyond it at 1.6, stream centres out to 60 kpc, and
+    streams switched off (|sigma| < 1e-4)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     base = synth.parameter_vectors(seed % 1000, p, count)
     S = p.number_streams
     for j in range(count):
         v = base[j]
         v[0] = rng.choice([0.3, 0.45, 0.7, 0.9, 1.0, 1.6])
-        v[1] = rng.choice([2.0, 4.0, 6.0, 8.0, 12.0])
+        v[1] = rng.choice([1.0, 2.0, 4.0, 8.0, 12.0, 20.0, 30.0])
         for s in range(S):
             o = 2 + 6 * s
-            v[o + 2] = rng.choice([2.0, 8.0, 25.0, 45.0, 60.0])      # r
-            v[o + 5] = rng.choice([0.6, 1.0, 2.5, 5.0, 12.0])        # sigma
+            v[o + 2] = rng.choice([0.7, 2.0, 8.0, 25.0, 45.0, 57.5, 60.0])      # r
+            v[o + 5] = rng.choice([0.1, 0.3, 0.6, 1.0, 2.5, 5.0, 12.0, 20.0])   # sigma
             if j % 11 == 5 and s == S - 1:
                 v[o + 5] = 5e-5                                       # switched off
     return base
