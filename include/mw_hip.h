@@ -179,6 +179,23 @@ int mw_ctx_region_integrals(const mw_ctx* ctx, double* out, size_t n);
 
 int mw_ctx_last_timing(const mw_ctx* ctx, mw_timing* out);
 
+/* One evaluation that allocates nothing.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * kernels as mw_ctx_eval for one parameter vector (params: the context's
+ * parameter count; its result is bitwise mw_ctx_eval's), MW_PREC_FP64 or
+ * MW_PREC_MIXED, models of at most 4 streams and 16 regions; no per-star
+ * output.  Its working memory (constants, work items, block partials,
+ * reduced rows) is a fixed array of MW_NOALLOC_SCRATCH_BYTES that every GPU
+ * the library runs on holds until the process ends; an evaluation that needs
+ * more (about 80 bytes per 256 sight columns and 112 per 256 stars) fails
+ * with MW_E_INVALID.  Calls on one GPU share that array: calls that overlap
+ * on one GPU race, so the caller serialises them.  stream: a hipStream_t, or
+ * NULL for the null stream; the call waits for it before returning. */
+#define MW_NOALLOC_SCRATCH_BYTES ((size_t)4 << 20)
+int mw_ctx_eval_noalloc(mw_ctx* ctx, const double* params, int precision, void* stream,
+                        mw_result* result);
+
 /* Device bytes the context holds resident. */
 size_t mw_ctx_resident_bytes(const mw_ctx* ctx);
 

@@ -23,7 +23,7 @@ LIB_PATH = LIB_DIR / LIB_NAME
 ARCH = os.environ.get("MW_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["mw_kernels.hip", "mw_api.hip", "mw_batch.hip", "mw_server.hip", "mw_separation.hip",
-               "mw_search.hip", "mw_quad.hip"]
+               "mw_search.hip", "mw_quad.hip", "mw_noalloc.hip"]
 HOST_SOURCES = ["mw_tables.cpp"]
 C_SOURCES = ["mw_io.c", "mw_search_host.c"]
 CLIENT_SOURCE = "client/mw_client.c"

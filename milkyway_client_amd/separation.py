@@ -52,7 +52,7 @@ EXPORTS = [
     "mw_get_step", "mw_get_min_parameters", "mw_get_max_parameters", "mw_hessian",
     "mw_ctx_hessian", "mw_matrix_invert", "mw_errors",
     "mw_write_checkpoint", "mw_read_checkpoint", "mw_calculate_progress",
-    "mw_resample_weights", "mw_resample_weights_padded",
+    "mw_resample_weights", "mw_resample_weights_padded", "mw_ctx_eval_noalloc",
 ]
 
 # int (*)(void* user, const double* vectors, int count, double* values)
