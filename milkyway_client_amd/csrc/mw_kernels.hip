@@ -84,47 +84,25 @@ __device__ __forceinline__ void block_reduce_dd(DD (&v)[NC], DD (*lds)[NC])
     }
 }
 
-// Sum one double per work-item over the workgroup in a fixed order (an xor
-// butterfly within each wave, whose partners add the same two values, then
-// the waves in order); result valid in thread 0.  The mixed kernels' block
-// partials: a few hundred positive terms per block, summed with a relative
-// error of a few ulp, the blocks then added in double-double by
-// reduce_kernel.
-template <int NC>
-__device__ __forceinline__ void block_reduce_f64(double (&v)[NC], double (*lds)[NC])
-{
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) {
-#pragma unroll
-        for (int c = 0; c < NC; c++) v[c] += __shfl_xor(v[c], m, 64);
-    }
-    const int wave = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-#pragma unroll
-        for (int c = 0; c < NC; c++) lds[wave][c] = v[c];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int c = 0; c < NC; c++) {
-            double a = lds[0][c];
-            for (int w = 1; w < kBlock / 64; w++) a += lds[w][c];
-            v[c] = a;
-        }
-    }
-}
-
-// The mixed integral's block result: components 0..S summed as above into
-// prow's double-double pairs (lo 0), the unused stream components 0.
+// The mixed integral's block result: components 0..S summed over the
+// workgroup in double-double (each work item's value enters as hi with lo
+// 0) into prow's (hi, lo) pairs, the unused stream components 0.
 template <int S>
-__device__ __forceinline__ void write_block_f64(double (&out)[1 + S], double* __restrict__ prow, void* red)
+__device__ __forceinline__ void write_block_dd(const double (&out)[1 + S], double* __restrict__ prow,
+                                               DD (*red)[kIntComps])
 {
-    block_reduce_f64<1 + S>(out, reinterpret_cast<double(*)[1 + S]>(red));
+    DD v[1 + S];
+#pragma unroll
+    for (int c = 0; c <= S; c++) v[c] = DD{ out[c], 0.0 };
+    block_reduce_dd<1 + S>(v, reinterpret_cast<DD(*)[1 + S]>(red));
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int c = 0; c < 2 * kIntComps; c++) prow[c] = 0.0;
 #pragma unroll
-        for (int c = 0; c <= S; c++) prow[2 * c] = out[c];
+        for (int c = 0; c <= S; c++) {
+            prow[2 * c] = v[c].hi;
+            prow[2 * c + 1] = v[c].lo;
+        }
     }
 }
 
@@ -518,7 +496,7 @@ __device__ __forceinline__ void integral_block_recurrence(const RegionLaunch& it
 #pragma unroll
         for (int s = 0; s < S; s++) out[1 + s] = acc[1 + s] * wnu * K.on[s] * exp2_neg(again.st[s].e);
     }
-    write_block_f64<S>(out, prow, red);
+    write_block_dd<S>(out, prow, red);
 }
 
 // One logical block of a resampled region in layout 4 (grid_step > 0,
@@ -560,14 +538,34 @@ __device__ __forceinline__ void integral_block_split(const RegionLaunch& it, lon
             const double p = sight.st[s].p, mm = sight.st[s].m;
             const double d = p * h;
             const double c = exp2_neg(2.0 * d * d);
-            // t rises along the sight, so a row's nodes all lie beyond the
-            // negligible bound T = sqrt(min t^2 + 70) exactly when t0 > T or
-            // t1 < -T; t0 advances by n d per row (a rounding per row, far
+            // Nodes with |t| above T = sqrt(min t^2 + kNegligibleLog2) hold
+            // terms below 2^-kNegligibleLog2 of the sight's largest; t0, the row's
+            // first t, advances by n d per row (a rounding per row, far
             // inside the bound's margin).
+            const double t_lo = fma(p, lo, -mm);
             const double floor_t2 =
-                min_t2(fma(p, lo, -mm), fma(p, fma((double)R * n - 1.0, h, lo), -mm)) + kNegligibleLog2;
+                min_t2(t_lo, fma(p, fma((double)R * n - 1.0, h, lo), -mm)) + kNegligibleLog2;
             const double T = floor_t2 * rsqrt_nr(floor_t2);
-            const double dn = d * n, d1 = d * (n - 1);
+            const double dn = d * n;
+            // t = t_lo + m d at node m rises along the sight, so the rows
+            // holding a node with |t| <= T form one interval [k_lo, k_hi] per
+            // lane (every row when d = 0, where |t| = |m| is within T); it is
+            // widened by a node at each end against rounding.  The wave runs
+            // the union of its lanes' intervals, so no row needs a test.
+            int k_lo = 0, k_hi = R - 1;
+            if (d > 0.0) {
+                const double id = rcp_nr(d);
+                const double m_a = fma(-T - t_lo, id, -1.0), m_b = fma(T - t_lo, id, 1.0);
+                k_lo = (int)fmin(fmax(floor(m_a / n), 0.0), (double)R);
+                k_hi = (int)fmin(fmax(floor(m_b / n), -1.0), (double)(R - 1));
+            }
+#pragma unroll
+            for (int m = 32; m >= 1; m >>= 1) {
+                k_lo = min(k_lo, __shfl_xor(k_lo, m, 64));
+                k_hi = max(k_hi, __shfl_xor(k_hi, m, 64));
+            }
+            const int k_first = __builtin_amdgcn_readfirstlane(k_lo);
+            const int k_last = __builtin_amdgcn_readfirstlane(k_hi);
             // Eight terms per step, in Horner form:
             //   a += G_j (w_j + R_j (w_{j+1} + R_{j+1} (... + R_{j+6} w_{j+7})))
             // with R_{j+k} = R_j c^k; then G_{j+8} = G_j P_j,
@@ -582,14 +580,9 @@ __device__ __forceinline__ void integral_block_split(const RegionLaunch& it, lon
                          c64 = (c16 * c16) * (c16 * c16);
             const ConstPtr wq = (ConstPtr)(it.st_rq[s]);
             double G = 0.0, Rr = 0.0, Pq = 0.0, a = 0.0;
-            double t0 = fma(p, lo, -mm);
+            double t0 = fma(dn, (double)k_first, t_lo);
             int since = kRestartRows;   // rows since the last restart: the first row restarts
-            for (int k = 0; k < R; k++, t0 += dn, since++) {
-                const double t1 = t0 + d1;
-                if (__all(t0 > T || t1 < -T)) {
-                    G = 0.0;   // negligible here; a later row before the peak restarts
-                    continue;
-                }
+            for (int k = k_first; k <= k_last; k++, t0 += dn, since++) {
                 if (since >= kRestartRows || (t0 < 0.0 && G < 0x1p-1000)) {
                     const double tc = fmax(t0, -40.0);
                     G = exp2_neg(tc * tc);
@@ -626,7 +619,7 @@ __device__ __forceinline__ void integral_block_split(const RegionLaunch& it, lon
 #pragma unroll
         for (int s = 0; s < S; s++) out[1 + s] = acc[1 + s] * wnu * K.on[s];
     }
-    write_block_f64<S>(out, prow, red);
+    write_block_dd<S>(out, prow, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -1017,28 +1010,12 @@ __device__ __forceinline__ void star_block(const StarLaunch& it, long lb,
     } else if (log10p && last) {
         log10p[(long)it.out_index * log10p_stride + idx] = lp;
     }
-    if constexpr (MIXED) {
-        // one double per component (every lo is 0 here), summed as the
-        // mixed integral's partials
-        double v[kStarComps];
+    block_reduce_dd<kStarComps>(out, red);
+    if (threadIdx.x == 0) {
 #pragma unroll
-        for (int c = 0; c < kStarComps; c++) v[c] = out[c].hi;
-        block_reduce_f64<kStarComps>(v, reinterpret_cast<double(*)[kStarComps]>(red));
-        if (threadIdx.x == 0) {
-#pragma unroll
-            for (int c = 0; c < kStarComps; c++) {
-                prow[2 * c] = v[c];
-                prow[2 * c + 1] = 0.0;
-            }
-        }
-    } else {
-        block_reduce_dd<kStarComps>(out, red);
-        if (threadIdx.x == 0) {
-#pragma unroll
-            for (int c = 0; c < kStarComps; c++) {
-                prow[2 * c] = out[c].hi;
-                prow[2 * c + 1] = out[c].lo;
-            }
+        for (int c = 0; c < kStarComps; c++) {
+            prow[2 * c] = out[c].hi;
+            prow[2 * c + 1] = out[c].lo;
         }
     }
 }
