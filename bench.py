@@ -436,8 +436,9 @@ def accuracy(got, ref) -> dict:
 
 
 def tune(groups, vset, precision, args, label):
-    if args.variant is not None and precision == sep.PREC_MIXED:
-        return {"variant": args.variant[0], "blocks_per_cu": args.variant[1], "table": [], "s": 0.0}
+    pinned = args.variant if precision == sep.PREC_MIXED else args.resampled_variant
+    if pinned is not None:
+        return {"variant": pinned[0], "blocks_per_cu": pinned[1], "table": [], "s": 0.0}
     t0 = time.perf_counter()
     choice = autotune.tune(groups, vset, precision, proxy_vectors=1, reps=1,
                            blocks_per_cu=(2, 4, 5), use_cache=False, log=log)
@@ -643,6 +644,8 @@ def main():
                     help="pin the headline batch's kernel variant and blocks per CU instead of "
                          "tuning (counter runs of one kernel: the tuner's timings move under "
                          "a profiler)")
+    ap.add_argument("--resampled-variant", type=int, nargs=2, metavar=("V", "BLOCKS_PER_CU"), default=None,
+                    help="pin the resampled batch's kernel variant and blocks per CU")
     ap.add_argument("--serving-requests", type=int, default=40)
     ap.add_argument("--serving-rates", type=float, nargs="+", default=[10.0, 50.0, 200.0, 800.0],
                     help="Poisson arrival rates (requests/s) of the serving sweep")

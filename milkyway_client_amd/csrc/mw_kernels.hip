@@ -84,25 +84,54 @@ __device__ __forceinline__ void block_reduce_dd(DD (&v)[NC], DD (*lds)[NC])
     }
 }
 
-// The mixed integral's block result: components 0..S summed over the
-// workgroup in double-double (each work item's value enters as hi with lo
-// 0) into prow's (hi, lo) pairs, the unused stream components 0.
-template <int S>
-__device__ __forceinline__ void write_block_dd(const double (&out)[1 + S], double* __restrict__ prow,
-                                               DD (*red)[kIntComps])
+// Sum one double per work-item over the workgroup in a fixed order (an xor
+// butterfly within each wave, whose partners add the same two values, then
+// the waves in order); result valid in thread 0.  For the resampled
+// kernels' block partials, positive terms only (a block's 256 column sums
+// of a few hundred positive grid terms): every term passes through 6
+// butterfly additions and at most 3 wave additions, so over positive terms
+// the result errs by at most 9 u of the block's sum (u = 2^-53, 1e-15
+// relative; tests/test_block_reduction.py); the blocks are then added in
+// double-double by
+// reduce_kernel.  Double-double here cost 14% of the resampled launch's
+// instructions (its blocks are short); the exact-quadrature kernels and the
+// star pass, whose blocks are long or whose terms change sign, reduce in
+// double-double.
+template <int NC>
+__device__ __forceinline__ void block_reduce_f64(double (&v)[NC], double (*lds)[NC])
 {
-    DD v[1 + S];
 #pragma unroll
-    for (int c = 0; c <= S; c++) v[c] = DD{ out[c], 0.0 };
-    block_reduce_dd<1 + S>(v, reinterpret_cast<DD(*)[1 + S]>(red));
+    for (int m = 32; m >= 1; m >>= 1) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) v[c] += __shfl_xor(v[c], m, 64);
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) lds[wave][c] = v[c];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            double a = lds[0][c];
+            for (int w = 1; w < kBlock / 64; w++) a += lds[w][c];
+            v[c] = a;
+        }
+    }
+}
+
+// The resampled integral's block result: components 0..S summed as above
+// into prow's (hi, lo) pairs (lo 0), the unused stream components 0.
+template <int S>
+__device__ __forceinline__ void write_block_f64(double (&out)[1 + S], double* __restrict__ prow, void* red)
+{
+    block_reduce_f64<1 + S>(out, reinterpret_cast<double(*)[1 + S]>(red));
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int c = 0; c < 2 * kIntComps; c++) prow[c] = 0.0;
 #pragma unroll
-        for (int c = 0; c <= S; c++) {
-            prow[2 * c] = v[c].hi;
-            prow[2 * c + 1] = v[c].lo;
-        }
+        for (int c = 0; c <= S; c++) prow[2 * c] = out[c];
     }
 }
 
@@ -496,7 +525,7 @@ __device__ __forceinline__ void integral_block_recurrence(const RegionLaunch& it
 #pragma unroll
         for (int s = 0; s < S; s++) out[1 + s] = acc[1 + s] * wnu * K.on[s] * exp2_neg(again.st[s].e);
     }
-    write_block_dd<S>(out, prow, red);
+    write_block_f64<S>(out, prow, red);
 }
 
 // One logical block of a resampled region in layout 4 (grid_step > 0,
@@ -559,13 +588,9 @@ __device__ __forceinline__ void integral_block_split(const RegionLaunch& it, lon
                 k_lo = (int)fmin(fmax(floor(m_a / n), 0.0), (double)R);
                 k_hi = (int)fmin(fmax(floor(m_b / n), -1.0), (double)(R - 1));
             }
-#pragma unroll
-            for (int m = 32; m >= 1; m >>= 1) {
-                k_lo = min(k_lo, __shfl_xor(k_lo, m, 64));
-                k_hi = max(k_hi, __shfl_xor(k_hi, m, 64));
-            }
-            const int k_first = __builtin_amdgcn_readfirstlane(k_lo);
-            const int k_last = __builtin_amdgcn_readfirstlane(k_hi);
+            // wavefront min / max (DPP reductions), made scalar
+            const int k_first = __builtin_amdgcn_readfirstlane(__ockl_wfred_min_i32(k_lo));
+            const int k_last = __builtin_amdgcn_readfirstlane(__ockl_wfred_max_i32(k_hi));
             // Eight terms per step, in Horner form:
             //   a += G_j (w_j + R_j (w_{j+1} + R_{j+1} (... + R_{j+6} w_{j+7})))
             // with R_{j+k} = R_j c^k; then G_{j+8} = G_j P_j,
@@ -619,7 +644,7 @@ __device__ __forceinline__ void integral_block_split(const RegionLaunch& it, lon
 #pragma unroll
         for (int s = 0; s < S; s++) out[1 + s] = acc[1 + s] * wnu * K.on[s];
     }
-    write_block_dd<S>(out, prow, red);
+    write_block_f64<S>(out, prow, red);
 }
 
 // ---------------------------------------------------------------------------
