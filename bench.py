@@ -338,6 +338,46 @@ def full_size_fp64(steps: int, warmup: int, precision=sep.PREC_FP64):
 SERVING_SET = 400
 
 
+def reference_size_fp64(steps: int = 20, warmup: int = 3) -> dict:
+    """One FP64 evaluation of the 70x160x64 workload (n = 30, 1 stream, 10k
+    stars): median over `steps` repetitions after `warmup`."""
+    p, stars = synth.workload(0, synth.REFERENCE_SIZE, 1)
+    with sep.Context(ParamsHolder(p), StarsHolder(stars)) as ctx:
+        v = p.parameter_vector()
+        for _ in range(warmup):
+            ctx.eval(v, precision=sep.PREC_FP64)
+        lat = []
+        for _ in range(steps):
+            s0 = time.perf_counter()
+            ctx.eval(v, precision=sep.PREC_FP64)
+            lat.append(time.perf_counter() - s0)
+    med = float(np.median(lat))
+    return {"value": round(1.0 / med, 2), "unit": "evals/s", "ms_per_eval_median": round(1e3 * med, 3),
+            "ms_per_eval_p99": round(1e3 * float(np.percentile(lat, 99)), 3),
+            "workload": "70x160x64 (r x mu x nu), n=30, 1 stream, 10k stars", "warmup": warmup,
+            "repetitions": steps}
+
+
+def full_size_mixed_32(reps: int = 3) -> dict:
+    """32 parameter vectors on the full-size wedge in one mixed-precision
+    evaluation call (the exact quadrature): evaluations per second, median
+    of `reps` calls after one warm-up."""
+    p, stars = synth.workload(1, synth.FULL_SIZE, synth.FULL_SIZE_STREAMS)
+    vecs = synth.parameter_vectors(2, p, 32)
+    with sep.Context(ParamsHolder(p), StarsHolder(stars)) as ctx:
+        ctx.eval(vecs, precision=sep.PREC_MIXED)
+        lat = []
+        for _ in range(reps):
+            s0 = time.perf_counter()
+            ctx.eval(vecs, precision=sep.PREC_MIXED)
+            lat.append(time.perf_counter() - s0)
+    med = float(np.median(lat))
+    return {"value": round(32 / med, 3), "unit": "evals/s", "s_per_call_median": round(med, 3),
+            "workload": ("32 vectors on 700x1600x640 + 700x400x160 cut, n=120, 3 streams, 200k stars, "
+                         "one mw_ctx_eval call, mixed precision, exact quadrature"),
+            "accuracy": "tests/test_gpu_configs.py::test_full_size_mixed_32_vectors (per star vs FP64)"}
+
+
 def serving_run(n_requests: int, rates, variant: int, bpc: int):
     """Poisson arrival streams at each rate over 8 resident wedges; requests
     are resampled mixed precision with a 10% FP64 share.  Every rate serves
@@ -562,6 +602,10 @@ def bench_batch(args):
         log("full-size FP64 evaluation (3 warm-ups, 20 repetitions)")
         _, _, full = full_size_fp64(20, 3)
         out["full_size_fp64"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in full.items()}
+        log("reference-size FP64 evaluation and 32 mixed vectors at full size")
+        out["reference_size_fp64"] = reference_size_fp64()
+        out["full_size_mixed_32"] = full_size_mixed_32()
+        log(f"  {out['reference_size_fp64']['value']} evals/s; {out['full_size_mixed_32']['value']} evals/s")
     if not args.no_serving:
         log("serving run")
         out["serving"] = serving_run(args.serving_requests, args.serving_rates, choice["variant"],
