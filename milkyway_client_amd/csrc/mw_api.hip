@@ -342,7 +342,8 @@ int run_plan(mw_ctx* c, Plan& P)
     if (timed) MW_HIP_CHECK(hipEventRecord(c->ev[0], st));
     for (const Group& g : igroups) {
         MW_HIP_CHECK(launch_integral(g.S, g.bg, P.mixed, c->ritems.p + g.first_item, g.n_items,
-                                     g.n_blocks, c->consts.p, c->ipart.p + (size_t)g.first_block * NI * 2, st));
+                                     g.n_blocks, c->consts.p, c->ipart.p + (size_t)g.first_block * NI * 2, st,
+                                     P.resample));
     }
     if (timed) MW_HIP_CHECK(hipEventRecord(c->ev[1], st));
     if (P.integrals && n_pr > 0) {

@@ -73,6 +73,7 @@ private:
         int bg;
         int first_item, n_items, first_block, n_blocks;
         bool split = true;   // every item in the split layout (integral groups)
+        bool exact = true;   // no item on a distance-grid recurrence (integral groups)
     };
     int device_ = 0, flags_ = 0, n_cu_ = 256;
     hipStream_t stream_ = nullptr;

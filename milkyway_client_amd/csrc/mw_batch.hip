@@ -188,6 +188,7 @@ int BatchRunner::plan(const std::vector<Slot>& slots, int precision)
             igroups_.push_back(Group{ bg, i, 0, total_iblocks_, 0 });
         Group& g = igroups_.back();
         g.split = g.split && mixed_ && L.grid_step > 0.0 && L.bg_stride == 0;
+        g.exact = g.exact && !(L.grid_step > 0.0);
         L.block_begin = g.n_blocks;
         hl[i] = L;
         hr[i] = ReduceItem{ total_iblocks_, L.block_count, L.out_index, 0 };
@@ -299,7 +300,8 @@ int BatchRunner::enqueue(bool kernel_events)
         MW_HIP_CHECK(launch_persistent(g.bg, mixed, variant, std::min(grid, g.n_blocks),
                                        ritems_.p + g.first_item, g.n_items, g.n_blocks, consts_.p,
                                        ipart_.p + (size_t)g.first_block * NI * 2,
-                                       counters_.p + gi, st, g.split));
+                                       counters_.p + gi, st,
+                                       g.split ? kLaunchSplitOnly : g.exact ? kLaunchExactOnly : kLaunchAny));
     }
     if (kernel_events) MW_HIP_CHECK(hipEventRecord(ev_[3], st));
     MW_HIP_CHECK(launch_reduce(false, rred_.p, n_ritems_, ipart_.p, irows_.p, st));

@@ -18,9 +18,10 @@
 
 namespace mw {
 
+// rec: some item may be on a distance-grid recurrence (grid_step > 0).
 hipError_t launch_integral(int S, int bg, bool mixed, const RegionLaunch* d_items, int n_items,
                            int n_blocks, const EvalConst* d_consts, double* d_partials,
-                           hipStream_t stream);
+                           hipStream_t stream, bool rec = true);
 hipError_t launch_star(int S, int bg, bool mixed, const StarLaunch* d_items, int n_items,
                        int n_blocks, const EvalConst* d_consts, const double* d_ratio,
                        const double* d_kw, double* d_partials, double* d_log10p, long stride,
@@ -29,12 +30,12 @@ hipError_t launch_reduce(bool star, const void* d_items, int n_items, const doub
                          double* d_rows, hipStream_t stream);
 hipError_t launch_finalize(const int* d_first, const int* d_count, const double* d_rows,
                            EvalConst* d_consts, int n_slots, hipStream_t stream);
-// split: every item is in the split layout (resampled, mixed, bg_stride 0),
-// which the split-only kernel then runs.
+// layouts: what the launch's items hold (kLaunchAny, kLaunchSplitOnly,
+// kLaunchExactOnly in mw_internal.h), which picks the kernel compiled for them.
 hipError_t launch_persistent(int bg, bool mixed, int variant, int grid, const RegionLaunch* d_items,
                              int n_items, int total_blocks, const EvalConst* d_consts,
                              double* d_partials, unsigned* d_counter, hipStream_t stream,
-                             bool split = false);
+                             int layouts = kLaunchAny);
 hipError_t launch_reset_counters(unsigned* d_counters, int n, hipStream_t stream);
 hipError_t launch_star_any(int bg, bool mixed, const StarLaunch* d_items, int n_items, int n_blocks,
                            const EvalConst* d_consts, double* d_partials, hipStream_t stream);

@@ -28,6 +28,13 @@ enum BgMode : int {
     kBgZeroQ = 3,         // q == 0: the background is -1 per r-step
 };
 
+// What the items of a persistent integral launch hold, which picks the
+// kernel compiled for them: kLaunchSplitOnly, every item in the split layout
+// (resampled, mixed, bg_stride 0); kLaunchExactOnly, no item on a
+// distance-grid recurrence (grid_step 0: exact tables, or resampled tables
+// with an exponential per node); kLaunchAny otherwise.
+constexpr int kLaunchAny = 0, kLaunchSplitOnly = 1, kLaunchExactOnly = 2;
+
 // Per-r-step table entry of one region (host-built, FP64).
 struct RStep {
     double irv;           // (r_{k+1}^3 - r_k^3) / 3 * dmu (radians)

@@ -379,11 +379,58 @@ __device__ __forceinline__ void point_terms(double r, double qw, const Sight<S>&
 //   * at a row that starts before the Gaussian's peak (t_0 < 0) with
 //     G < 2^-1000: such a G may have lost precision to underflow, and the
 //     terms are about to grow.
-// Past the peak the terms only shrink, so underflow there loses nothing.  The
-// caller guarantees d <= 1/2; t_0 is clamped at -40 on restart, so R stays
-// below 2^40, and a clamped start gives G = 2^-1600 = 0, which stays 0 until
-// the next restart (the terms it stands for are below 2^-(40 - 15.5)^2).
+// Past the peak the terms only shrink, so underflow there loses nothing.
+// The terms are carried relative to the sight's largest on-grid term: with
+// s^2 the smallest t^2 over the grid's nodes (0 when the Gaussian's peak
+// lies on the grid), G_j = 2^-(t_j^2 - s^2) <= 1, and the sight's sum is
+// scaled by 2^-(e + s^2) at the end.  Without the shift, a sight whose peak
+// lies beyond the grid would start its on-grid terms below 2^-1000, in
+// subnormal numbers or at 0.
+// A restart can still fall on a node far below that largest term: a wave
+// runs the union of its lanes' rows, so a lane may start rows before its own
+// terms matter, and a row of 32 nodes spans up to 16 in t.  A G started in
+// subnormal numbers carries its rounding (up to 100 %) to every later term,
+// the peak's included, until the next restart.  So G and the sight's running
+// sum are carried scaled by 2^B, B a whole number >= 0 set at each restart:
+// B = ceil(t_0^2 - s^2 - 1000) when the start lies before the peak and more
+// than 2^-1000 below the largest term, else 0.  G then starts in [2^-1000, 2^-999), with full
+// precision; the scaled G stays below 2^B <= 2^600 and the scaled sum below
+// 2^610, and rescaling by a power of two is exact.  The sum is unscaled
+// (2^-B) at the end.  The caller guarantees d <= 1/2.  On restart t_0 is
+// clamped at -sqrt(s^2 + 1600), so R stays below 2^(sqrt(s^2 + 1600)), and a
+// clamped start sets G = 0, which stays 0 until the next restart: the terms
+// of a row that starts there lie below 2^-((sqrt(s^2 + 1600) - 16)^2 - s^2),
+// 2^-190 or less, of the largest.  A sight whose s^2 reaches 1100 has every
+// on-grid term below 2^-1100 of its peak, so its final factor 2^-(e + s^2)
+// is 0: its recurrence is not run (G = R = 0).
 constexpr int kRestartRows = 8;
+constexpr double kRestartClampT2 = 1600.0;
+constexpr double kGoneT2 = 1100.0;
+constexpr double kScaleT2 = 1000.0;
+
+// Restart of the recurrence at a row's first node t0 (with spacing d), terms
+// relative to 2^-(s^2) and scaled by 2^B: G = 2^(B - (t0^2 - s^2)),
+// R = 2^-(2 t0 d + d^2), B and the running sum a rescaled as above, t0
+// clamped as above; G = R = 0 for a sight whose s^2 >= kGoneT2.
+__device__ __forceinline__ void restart_recurrence(double t0, double d, double s2, double& G, double& R,
+                                                   int& B, double& a)
+{
+    if (s2 >= kGoneT2) {
+        G = R = 0.0;
+        return;
+    }
+    const double lim2 = s2 + kRestartClampT2;
+    const double lim = lim2 * rsqrt_nr(lim2);
+    const double tc = fmax(t0, -lim);
+    R = exp2_neg(fma(2.0 * tc, d, d * d));
+    const double D = fma(tc, tc, -s2);   // t0 is a grid node: D >= 0 up to rounding
+    // before the peak only (t0 < 0, where the clamp bounds D by 1600): past
+    // it the terms only shrink and a deep start is negligible as it stands
+    const int Bn = (t0 < 0.0 && D > kScaleT2) ? (int)ceil(D - kScaleT2) : 0;
+    a = ldexp(a, Bn - B);
+    B = Bn;
+    G = t0 > -lim ? exp2_neg(D - (double)Bn) : 0.0;
+}
 
 // A stream's terms in a row are skipped, by the whole wave, when every one of
 // them on every lane lies below 2^-kNegligibleLog2 of that sight's largest
@@ -405,9 +452,10 @@ template <int S, int BG, int Q>
 __device__ __forceinline__ void row_by_recurrence(const double2* row, int n, double r0, double h,
                                                   bool restart, const Sight<S>& v,
                                                   const double (&C)[S > 0 ? S : 1],
-                                                  const float (&floor_t2)[S > 0 ? S : 1],
+                                                  const double (&s2)[S > 0 ? S : 1],
                                                   double (&G)[S > 0 ? S : 1],
-                                                  double (&Rr)[S > 0 ? S : 1], const EvalConst& K,
+                                                  double (&Rr)[S > 0 ? S : 1],
+                                                  int (&B)[S > 0 ? S : 1], const EvalConst& K,
                                                   double (&acc)[S + 1])
 {
     if constexpr (Q == 1) {
@@ -432,16 +480,13 @@ __device__ __forceinline__ void row_by_recurrence(const double2* row, int n, dou
         const double d = p * h;
         const double t0 = fma(p, r0, -v.st[s].m);
         const double t1 = fma(d, (double)(n - 1), t0);
-        if (__all(min_t2(t0, t1) > (double)floor_t2[s])) {
+        if (__all(min_t2(t0, t1) > s2[s] + kNegligibleLog2)) {
             // Negligible here; a later row before the peak restarts from G = 0.
             G[s] = 0.0;
             continue;
         }
-        if (restart || (t0 < 0.0 && G[s] < 0x1p-1000)) {
-            const double tc = fmax(t0, -40.0);
-            G[s] = exp2_neg(tc * tc);
-            Rr[s] = exp2_neg(fma(2.0 * tc, d, d * d));
-        }
+        if (restart || (t0 < 0.0 && G[s] < 0x1p-1000))
+            restart_recurrence(t0, d, s2[s], G[s], Rr[s], B[s], acc[1 + s]);
         double g = G[s], rr = Rr[s], a = acc[1 + s];
         const double c = C[s];
 #pragma unroll 4
@@ -480,21 +525,25 @@ __device__ __forceinline__ void integral_block_recurrence(const RegionLaunch& it
     const double h = it.grid_step;
 
     double acc[1 + S];
+    double s2[S > 0 ? S : 1];
+    int B[S > 0 ? S : 1];   // each stream's sum carried scaled by 2^B (restart_recurrence)
 #pragma unroll
     for (int c = 0; c <= S; c++) acc[c] = 0.0;
     {
         const Sight<S> sight = make_sight<S, BG, true>(it.col[cc], it.col[C + cc], it.col[2 * C + cc], K);
         double Cs[S > 0 ? S : 1], G[S > 0 ? S : 1], Rr[S > 0 ? S : 1];
-        float floor_t2[S > 0 ? S : 1];   // rows whose t^2 all exceed this are negligible
+        // s2: each stream's smallest t^2 over the grid's nodes (the terms are
+        // carried relative to 2^-s2; rows whose t^2 all exceed s2 + 60 are
+        // negligible)
         const double r_last = fma((double)R * n - 1.0, h, it.grid_lo);
 #pragma unroll
         for (int s = 0; s < S; s++) {
             const double d = sight.st[s].p * h;
             Cs[s] = exp2_neg(2.0 * d * d);
             G[s] = Rr[s] = 0.0;
-            const double lo = min_t2(fma(sight.st[s].p, it.grid_lo, -sight.st[s].m),
-                                     fma(sight.st[s].p, r_last, -sight.st[s].m));
-            floor_t2[s] = (float)(lo + kNegligibleLog2);   // rounding either way moves the bound by < 1e-5
+            B[s] = 0;
+            s2[s] = min_t2(fma(sight.st[s].p, it.grid_lo, -sight.st[s].m),
+                           fma(sight.st[s].p, r_last, -sight.st[s].m));
         }
         const int rt = kTileD2 / n;
         const double2* rq = reinterpret_cast<const double2*>(it.rq);
@@ -507,7 +556,7 @@ __device__ __forceinline__ void integral_block_recurrence(const RegionLaunch& it
                 const int k = k0 + kk;
                 const double r0 = fma((double)k * n, h, it.grid_lo);
                 row_by_recurrence<S, BG, Q>(tile + kk * n, n, r0, h, k % kRestartRows == 0, sight, Cs,
-                                            floor_t2, G, Rr, K, acc);
+                                            s2, G, Rr, B, K, acc);
                 if constexpr (BG == kBgHernquistAux) {
                     const RStep& rs = it.rstep[k];
                     acc[0] += K.bg_a * rs.q2 + K.bg_b * rs.q1 + K.bg_c * rs.q0;
@@ -523,7 +572,8 @@ __device__ __forceinline__ void integral_block_recurrence(const RegionLaunch& it
         const Sight<S> again = make_sight<S, BG, true>(it.col[cc], it.col[C + cc], it.col[2 * C + cc], K);
         out[0] = acc[0] * wnu;
 #pragma unroll
-        for (int s = 0; s < S; s++) out[1 + s] = acc[1 + s] * wnu * K.on[s] * exp2_neg(again.st[s].e);
+        for (int s = 0; s < S; s++)
+            out[1 + s] = ldexp(acc[1 + s], -B[s]) * wnu * K.on[s] * exp2_neg(again.st[s].e + s2[s]);
     }
     write_block_f64<S>(out, prow, red);
 }
@@ -572,8 +622,9 @@ __device__ __forceinline__ void integral_block_split(const RegionLaunch& it, lon
             // first t, advances by n d per row (a rounding per row, far
             // inside the bound's margin).
             const double t_lo = fma(p, lo, -mm);
-            const double floor_t2 =
-                min_t2(t_lo, fma(p, fma((double)R * n - 1.0, h, lo), -mm)) + kNegligibleLog2;
+            // the smallest t^2 over the grid's nodes: terms are carried relative to 2^-s2
+            const double s2 = min_t2(t_lo, fma(p, fma((double)R * n - 1.0, h, lo), -mm));
+            const double floor_t2 = s2 + kNegligibleLog2;
             const double T = floor_t2 * rsqrt_nr(floor_t2);
             const double dn = d * n;
             // t = t_lo + m d at node m rises along the sight, so the rows
@@ -605,13 +656,12 @@ __device__ __forceinline__ void integral_block_split(const RegionLaunch& it, lon
                          c64 = (c16 * c16) * (c16 * c16);
             const ConstPtr wq = (ConstPtr)(it.st_rq[s]);
             double G = 0.0, Rr = 0.0, Pq = 0.0, a = 0.0;
+            int B = 0;   // G and a carried scaled by 2^B (restart_recurrence)
             double t0 = fma(dn, (double)k_first, t_lo);
             int since = kRestartRows;   // rows since the last restart: the first row restarts
             for (int k = k_first; k <= k_last; k++, t0 += dn, since++) {
                 if (since >= kRestartRows || (t0 < 0.0 && G < 0x1p-1000)) {
-                    const double tc = fmax(t0, -40.0);
-                    G = exp2_neg(tc * tc);
-                    Rr = exp2_neg(fma(2.0 * tc, d, d * d));
+                    restart_recurrence(t0, d, s2, G, Rr, B, a);
                     const double q = Rr * Rr * c, q2 = q * q;
                     Pq = (q2 * q2) * c24;   // (R^2 c)^4 c^24 = R^8 c^28
                     since = 0;
@@ -632,7 +682,8 @@ __device__ __forceinline__ void integral_block_split(const RegionLaunch& it, lon
                     Rr *= c8;
                 }
             }
-            acc[1 + s] = a * exp2_neg(sight.st[s].e);   // the sight's minimum, taken out of every term
+            // the scales taken out of every term
+            acc[1 + s] = ldexp(a, -B) * exp2_neg(sight.st[s].e + s2);
         }
     }
     double out[1 + S];
@@ -651,13 +702,16 @@ __device__ __forceinline__ void integral_block_split(const RegionLaunch& it, lon
 // Integral: one logical block = 256 consecutive columns of one region of one
 // evaluation slot; the block's double-double partials go to prow.
 // ---------------------------------------------------------------------------
-template <int S, int BG, bool MIXED, int UNROLL>
+// REC: the mixed kernels' recurrence and split layouts compiled in (items
+// with grid_step > 0); without them the kernel holds the exact tables' loop
+// alone, and its register allocation is that loop's.
+template <int S, int BG, bool MIXED, int UNROLL, bool REC = true>
 __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
                                                const EvalConst* __restrict__ consts,
                                                double* __restrict__ prow, double2* tile,
                                                DD (*red)[kIntComps])
 {
-    if constexpr (MIXED) {
+    if constexpr (MIXED && REC) {
         if (it.grid_step > 0.0) {
             if (it.bg_stride == 0) integral_block_split<S, BG>(it, lb, consts, prow, red);
             else if (it.bg_stride == 4) integral_block_recurrence<S, BG, 4>(it, lb, consts, prow, tile, red);
@@ -748,8 +802,8 @@ __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
 }
 
 // One hardware block per logical block; the stream count is a template
-// parameter shared by every item of the launch.
-template <int S, int BG, bool MIXED>
+// parameter shared by every item of the launch (REC as for integral_block).
+template <int S, int BG, bool MIXED, bool REC = true>
 __global__ __launch_bounds__(kBlock) void integral_kernel(const RegionLaunch* __restrict__ items,
                                                           int n_items,
                                                           const EvalConst* __restrict__ consts,
@@ -759,22 +813,22 @@ __global__ __launch_bounds__(kBlock) void integral_kernel(const RegionLaunch* __
     __shared__ DD red[kBlock / 64][kIntComps];
     const int b = blockIdx.x;
     const RegionLaunch& it = items[find_item(items, n_items, b)];
-    integral_block<S, BG, MIXED, 2>(it, b - it.block_begin, consts,
-                                    partials + (long)b * kIntComps * 2, tile, red);
+    integral_block<S, BG, MIXED, 2, REC>(it, b - it.block_begin, consts,
+                                         partials + (long)b * kIntComps * 2, tile, red);
 }
 
-template <int BG, bool MIXED, int UNROLL>
+template <int BG, bool MIXED, int UNROLL, bool REC>
 __device__ __forceinline__ void integral_block_any(const RegionLaunch& it, long lb,
                                                    const EvalConst* __restrict__ consts,
                                                    double* __restrict__ prow, double2* tile,
                                                    DD (*red)[kIntComps])
 {
     switch (it.n_streams) {
-    case 0: integral_block<0, BG, MIXED, UNROLL>(it, lb, consts, prow, tile, red); break;
-    case 1: integral_block<1, BG, MIXED, UNROLL>(it, lb, consts, prow, tile, red); break;
-    case 2: integral_block<2, BG, MIXED, UNROLL>(it, lb, consts, prow, tile, red); break;
-    case 3: integral_block<3, BG, MIXED, UNROLL>(it, lb, consts, prow, tile, red); break;
-    default: integral_block<4, BG, MIXED, UNROLL>(it, lb, consts, prow, tile, red); break;
+    case 0: integral_block<0, BG, MIXED, UNROLL, REC>(it, lb, consts, prow, tile, red); break;
+    case 1: integral_block<1, BG, MIXED, UNROLL, REC>(it, lb, consts, prow, tile, red); break;
+    case 2: integral_block<2, BG, MIXED, UNROLL, REC>(it, lb, consts, prow, tile, red); break;
+    case 3: integral_block<3, BG, MIXED, UNROLL, REC>(it, lb, consts, prow, tile, red); break;
+    default: integral_block<4, BG, MIXED, UNROLL, REC>(it, lb, consts, prow, tile, red); break;
     }
 }
 
@@ -783,8 +837,10 @@ __device__ __forceinline__ void integral_block_any(const RegionLaunch& it, long 
 // block, largest first) until every block is taken; every wave then exits.
 // Items of any stream count share the launch.  Results are written per
 // logical block, so they do not depend on which hardware block ran them.
-// WAVES is the amdgpu waves-per-EU occupancy target (1 = compiler default).
-template <int BG, bool MIXED, int UNROLL, int WAVES>
+// WAVES is the amdgpu waves-per-EU occupancy target (1 = compiler default);
+// REC false: every item is on exact tables or an exponential-per-node layout
+// (grid_step 0), so the recurrence paths are left out.
+template <int BG, bool MIXED, int UNROLL, int WAVES, bool REC = true>
 __global__ __launch_bounds__(kBlock, WAVES) void integral_persistent(
     const RegionLaunch* __restrict__ items, int n_items, int total_blocks,
     const EvalConst* __restrict__ consts, double* __restrict__ partials, unsigned* counter)
@@ -799,8 +855,8 @@ __global__ __launch_bounds__(kBlock, WAVES) void integral_persistent(
         const int b = next;
         if (b >= total_blocks) break;
         const RegionLaunch& it = items[find_item(items, n_items, b)];
-        integral_block_any<BG, MIXED, UNROLL>(it, b - it.block_begin, consts,
-                                              partials + (long)b * kIntComps * 2, tile, red);
+        integral_block_any<BG, MIXED, UNROLL, REC>(it, b - it.block_begin, consts,
+                                                   partials + (long)b * kIntComps * 2, tile, red);
     }
 }
 
@@ -1227,15 +1283,15 @@ using IntegralFn = void (*)(const RegionLaunch*, int, const EvalConst*, double*)
 using StarFn = void (*)(const StarLaunch*, int, const EvalConst*, const double*, const double*,
                         double*, double*, long);
 
-template <bool MIXED, int BG>
+template <bool MIXED, int BG, bool REC = true>
 IntegralFn integral_fn(int S)
 {
     switch (S) {
-    case 0: return integral_kernel<0, BG, MIXED>;
-    case 1: return integral_kernel<1, BG, MIXED>;
-    case 2: return integral_kernel<2, BG, MIXED>;
-    case 3: return integral_kernel<3, BG, MIXED>;
-    default: return integral_kernel<4, BG, MIXED>;
+    case 0: return integral_kernel<0, BG, MIXED, REC>;
+    case 1: return integral_kernel<1, BG, MIXED, REC>;
+    case 2: return integral_kernel<2, BG, MIXED, REC>;
+    case 3: return integral_kernel<3, BG, MIXED, REC>;
+    default: return integral_kernel<4, BG, MIXED, REC>;
     }
 }
 
@@ -1251,11 +1307,13 @@ StarFn star_fn(int S)
     }
 }
 
-// Mixed precision is instantiated for the Hernquist profile; the other
-// background modes run the FP64 kernels.
-static IntegralFn pick_integral(int S, int bg, bool mixed)
+// Mixed precision is instantiated for the Hernquist profile (with the
+// recurrence layouts, rec, or without); the other background modes run the
+// FP64 kernels.
+static IntegralFn pick_integral(int S, int bg, bool mixed, bool rec)
 {
-    if (mixed && bg == kBgHernquist) return integral_fn<true, kBgHernquist>(S);
+    if (mixed && bg == kBgHernquist)
+        return rec ? integral_fn<true, kBgHernquist, true>(S) : integral_fn<true, kBgHernquist, false>(S);
     switch (bg) {
     case kBgHernquist: return integral_fn<false, kBgHernquist>(S);
     case kBgHernquistAux: return integral_fn<false, kBgHernquistAux>(S);
@@ -1301,10 +1359,10 @@ static SeparationFn pick_separation(int S, int bg, bool mixed)
 
 hipError_t launch_integral(int S, int bg, bool mixed, const RegionLaunch* d_items, int n_items,
                            int n_blocks, const EvalConst* d_consts, double* d_partials,
-                           hipStream_t stream)
+                           hipStream_t stream, bool rec)
 {
     if (n_blocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(pick_integral(S, bg, mixed), dim3(n_blocks), dim3(kBlock), 0, stream,
+    hipLaunchKernelGGL(pick_integral(S, bg, mixed, rec), dim3(n_blocks), dim3(kBlock), 0, stream,
                        d_items, n_items, d_consts, d_partials);
     return hipGetLastError();
 }
@@ -1366,31 +1424,33 @@ constexpr int kNumVariants = 6;
 
 using PersistentFn = void (*)(const RegionLaunch*, int, int, const EvalConst*, double*, unsigned*);
 
-template <int BG, bool MIXED, int V>
+template <int BG, bool MIXED, int V, bool REC = true>
 PersistentFn persistent_variant()
 {
-    return integral_persistent<BG, MIXED, kVariantUnroll[V], kVariantWaves[V]>;
+    return integral_persistent<BG, MIXED, kVariantUnroll[V], kVariantWaves[V], REC>;
 }
 
-template <int BG, bool MIXED>
+template <int BG, bool MIXED, bool REC = true>
 PersistentFn persistent_fn(int v)
 {
     switch (v) {
-    case 1: return persistent_variant<BG, MIXED, 1>();
-    case 2: return persistent_variant<BG, MIXED, 2>();
-    case 3: return persistent_variant<BG, MIXED, 3>();
-    case 4: return persistent_variant<BG, MIXED, 4>();
-    case 5: return persistent_variant<BG, MIXED, 5>();
-    default: return persistent_variant<BG, MIXED, 0>();
+    case 1: return persistent_variant<BG, MIXED, 1, REC>();
+    case 2: return persistent_variant<BG, MIXED, 2, REC>();
+    case 3: return persistent_variant<BG, MIXED, 3, REC>();
+    case 4: return persistent_variant<BG, MIXED, 4, REC>();
+    case 5: return persistent_variant<BG, MIXED, 5, REC>();
+    default: return persistent_variant<BG, MIXED, 0, REC>();
     }
 }
 
-// Every variant for the Hernquist profile in both precisions; the other
-// background modes run variant 0 in FP64.
-static PersistentFn pick_persistent(int bg, bool mixed, int v)
+// Every variant for the Hernquist profile in both precisions (the mixed one
+// with and without the recurrence paths); the other background modes run
+// variant 0 in FP64.
+static PersistentFn pick_persistent(int bg, bool mixed, int v, bool rec)
 {
-    if (bg == kBgHernquist) return mixed ? persistent_fn<kBgHernquist, true>(v)
-                                         : persistent_fn<kBgHernquist, false>(v);
+    if (bg == kBgHernquist) return !mixed ? persistent_fn<kBgHernquist, false>(v)
+                                   : rec ? persistent_fn<kBgHernquist, true, true>(v)
+                                         : persistent_fn<kBgHernquist, true, false>(v);
     switch (bg) {
     case kBgHernquistAux: return persistent_variant<kBgHernquistAux, false, 0>();
     case kBgGeneral: return persistent_variant<kBgGeneral, false, 0>();
@@ -1410,11 +1470,12 @@ static PersistentFn pick_persistent_split(int v)
 
 hipError_t launch_persistent(int bg, bool mixed, int variant, int grid, const RegionLaunch* d_items,
                              int n_items, int total_blocks, const EvalConst* d_consts,
-                             double* d_partials, unsigned* d_counter, hipStream_t stream, bool split)
+                             double* d_partials, unsigned* d_counter, hipStream_t stream, int layouts)
 {
     if (total_blocks <= 0) return hipSuccess;
-    const PersistentFn fn = (split && mixed && bg == kBgHernquist) ? pick_persistent_split(variant)
-                                                                    : pick_persistent(bg, mixed, variant);
+    const PersistentFn fn = (layouts == kLaunchSplitOnly && mixed && bg == kBgHernquist)
+                                ? pick_persistent_split(variant)
+                                : pick_persistent(bg, mixed, variant, layouts != kLaunchExactOnly);
     hipLaunchKernelGGL(fn, dim3(grid), dim3(kBlock), 0, stream, d_items, n_items, total_blocks, d_consts,
                        d_partials, d_counter);
     return hipGetLastError();

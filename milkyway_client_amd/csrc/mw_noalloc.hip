@@ -175,7 +175,7 @@ extern "C" int mw_ctx_eval_noalloc(mw_ctx* c, const double* params, int precisio
     MW_HIP_CHECK(hipMemcpyAsync(d_items, items, sizeof(RegionLaunch) * G, hipMemcpyHostToDevice, st));
     MW_HIP_CHECK(hipMemcpyAsync(d_ired, ired, sizeof(ReduceItem) * G, hipMemcpyHostToDevice, st));
     MW_HIP_CHECK(hipMemcpyAsync(d_first, two, sizeof(two), hipMemcpyHostToDevice, st));
-    MW_HIP_CHECK(launch_integral(S, K.bg_mode, mixed, d_items, G, iblocks, d_k, d_ipart, st));
+    MW_HIP_CHECK(launch_integral(S, K.bg_mode, mixed, d_items, G, iblocks, d_k, d_ipart, st, false));
     MW_HIP_CHECK(launch_reduce(false, d_ired, G, d_ipart, d_irows, st));
     if (sblocks > 0) {
         MW_HIP_CHECK(hipMemcpyAsync(d_sitem, &sitem, sizeof(sitem), hipMemcpyHostToDevice, st));
