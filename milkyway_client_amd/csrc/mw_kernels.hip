@@ -392,13 +392,13 @@ __device__ __forceinline__ void point_terms(double r, double qw, const Sight<S>&
 // subnormal numbers carries its rounding (up to 100 %) to every later term,
 // the peak's included, until the next restart.  So G and the sight's running
 // sum are carried scaled by 2^B, B a whole number >= 0 set at each restart:
-// B = ceil(t_0^2 - s^2 - 1000) when the start lies before the peak and more
-// than 2^-1000 below the largest term, else 0.  G then starts in [2^-1000, 2^-999), with full
+// B = ceil(t_0^2 - s^2 - 1000) when the start lies before the peak, short of
+// the clamp below, and more than 2^-1000 below the largest term, else 0.  G then starts in [2^-1000, 2^-999), with full
 // precision; the scaled G stays below 2^B <= 2^600 and the scaled sum below
 // 2^610, and rescaling by a power of two is exact.  The sum is unscaled
 // (2^-B) at the end.  The caller guarantees d <= 1/2.  On restart t_0 is
 // clamped at -sqrt(s^2 + 1600), so R stays below 2^(sqrt(s^2 + 1600)), and a
-// clamped start sets G = 0, which stays 0 until the next restart: the terms
+// clamped start gives G = 2^-1600 = 0, which stays 0 until the next restart: the terms
 // of a row that starts there lie below 2^-((sqrt(s^2 + 1600) - 16)^2 - s^2),
 // 2^-190 or less, of the largest.  A sight whose s^2 reaches 1100 has every
 // on-grid term below 2^-1100 of its peak, so its final factor 2^-(e + s^2)
@@ -422,14 +422,18 @@ __device__ __forceinline__ void restart_recurrence(double t0, double d, double s
     const double lim2 = s2 + kRestartClampT2;
     const double lim = lim2 * rsqrt_nr(lim2);
     const double tc = fmax(t0, -lim);
-    R = exp2_neg(fma(2.0 * tc, d, d * d));
     const double D = fma(tc, tc, -s2);   // t0 is a grid node: D >= 0 up to rounding
-    // before the peak only (t0 < 0, where the clamp bounds D by 1600): past
-    // it the terms only shrink and a deep start is negligible as it stands
-    const int Bn = (t0 < 0.0 && D > kScaleT2) ? (int)ceil(D - kScaleT2) : 0;
+    // Scaled only before the peak and short of the clamp: past the peak the
+    // terms only shrink and a deep start is negligible as it stands; a
+    // clamped start (D = 1600) keeps B = 0, so that its G = 2^-1600 is 0.
+    const double over = ceil(D - kScaleT2);   // > 0 exactly when D > kScaleT2
+    const int Bn = (t0 < 0.0 && t0 > -lim && over > 0.0) ? (int)over : 0;
+    // both exponentials side by side, unconditionally (they share the
+    // polynomial's constants)
+    R = exp2_neg(fma(2.0 * tc, d, d * d));
+    G = exp2_neg(D - (double)Bn);
     a = ldexp(a, Bn - B);
     B = Bn;
-    G = t0 > -lim ? exp2_neg(D - (double)Bn) : 0.0;
 }
 
 // A stream's terms in a row are skipped, by the whole wave, when every one of

@@ -4,8 +4,9 @@ It is mixed precision on a quadrature whose weights are folded onto a grid
 in distance or magnitude.  Bars, as for the mixed path: every star's |delta ln P| <= 1e-7
 against the FP64 oracle, integrals within 2e-8 relative (the FP32
 transcendentals dominate; the quadrature change itself is ~1e-10).  Cases
-include the narrowest streams the parameter ranges allow (sigma = 1 kpc),
-all four stream counts, the aux profile, the full-size wedge against the
+include the narrowest streams the parameter ranges allow (sigma = 0.1 kpc),
+the corners of the reference's parameter bounds against the oracle, every
+stream count up to four, the aux profile, the full-size wedge against the
 GPU's own FP64 result, and bitwise equality between single, batched
 (eager and graph) and served evaluations.
 """
@@ -92,6 +93,26 @@ def test_narrowest_streams_far_out(sigma):
     e, grew = errors_vs_oracle(p, stars)
     assert grew
     assert e["lnP"] <= LNP_TOL and e["I_st"] <= INT_REL_TOL and e["I_bg"] <= INT_REL_TOL
+
+
+@pytest.mark.parametrize("q", [0.3, 1.0])
+@pytest.mark.parametrize("r0", [1.0, 30.0])
+@pytest.mark.parametrize("sigma", [0.1, 20.0])
+def test_reference_bounds_against_oracle(q, r0, sigma):
+    """The corners of the reference's parameter bounds (background_min /
+    background_max: q 0.3-1.0, r0 1-30 kpc; stream_min / stream_max: sigma
+    0.1-20 kpc, tests/astronomy_parameters-11-small.txt and -86-small.txt),
+    every stream at sigma, on the full r and convolution resolution, against
+    the oracle: every star and every integral within the bars."""
+    p, stars = synth.workload(301, LONG_R, 3)
+    p.background_parameters[1] = q
+    p.background_parameters[2] = r0
+    for s in p.streams:
+        s.parameters[4] = sigma
+    e, grew = errors_vs_oracle(p, stars)
+    assert grew, "the resampled grid was not used"
+    assert e["lnP"] <= LNP_TOL and e["lnL"] <= LNP_TOL
+    assert e["I_bg"] <= INT_REL_TOL and e["I_st"] <= INT_REL_TOL
 
 
 @pytest.mark.parametrize("field,value", [("sigma", 0.05), ("r0", 0.5), ("r0", 40.0), ("q", 0.2),

@@ -10,8 +10,8 @@ R_{j+k} = R_j c^k, then G_{j+8} = G_j P_j with P_j = R_j^8 c^28,
 P_{j+8} = P_j c^64 and R_{j+8} = R_j c^8, restarting from the exact 2^x every
 8 rows of 32 nodes and at a row that starts before the peak with G < 2^-1000
 (t clamped at -sqrt(s^2 + 1600) on restart), G and the sum carried scaled
-by 2^B (B = ceil(t^2 - s^2 - 1000) at a restart before the peak more than
-2^-1000 below the largest term, else 0), and running no terms for a sight with s^2 >= 1100.  Over that interval the shifted sum must stay
+by 2^B (B = ceil(t^2 - s^2 - 1000) at a restart before the peak, short of
+the clamp, more than 2^-1000 below the largest term, else 0), and running no terms for a sight with s^2 >= 1100.  Over that interval the shifted sum must stay
 within 4e-12 of the direct one for any start and spacing the kernel allows
 (d <= 1/2, t_0 from -60 to 5: the Gaussian's peak ahead of the interval, on
 it, or behind it; starts in subnormal range and beyond the clamp).  A start P_0 of R^8 c^24 (one factor c^4 short) must
@@ -54,10 +54,11 @@ def grouped(w, t0, d, s2, short_start=False, underflow_restart=True, scale=True)
             tc = max(t_row, -lim)
             R = 2.0 ** (-(2 * tc * d + d * d))
             D = tc * tc - s2
-            Bn = int(np.ceil(D - 1000.0)) if (scale and t_row < 0.0 and D > 1000.0) else 0
+            deep = scale and -lim < t_row < 0.0 and D > 1000.0
+            Bn = int(np.ceil(D - 1000.0)) if deep else 0
             a = float(np.ldexp(a, Bn - B))
             B = Bn
-            G = 2.0 ** (-(D - Bn)) if t_row > -lim else 0.0
+            G = 2.0 ** max(-(D - Bn), -1100.0)   # a clamped start: 2^-1600 held at 2^-1100 = 0
             q = R * R * c
             q2 = q * q
             P = (q2 * q2) * (c16 * c4 if short_start else c24)

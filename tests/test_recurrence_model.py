@@ -7,10 +7,12 @@ column's nodes (0 when the Gaussian's peak lies on the grid): the terms are
 carried relative to the column's largest, and the column's sum is scaled by
 2^-s^2 at the end.  The recurrence restarts from exact values every 8 rows of
 32 nodes and at any row that starts before the peak (t < 0) with
-G < 2^-1000, t clamped at -sqrt(s^2 + 1600) on restart (G = 0 when clamped);
+G < 2^-1000, t clamped at -sqrt(s^2 + 1600) on restart (G = 2^-1600 = 0 when
+clamped);
 G and the column's running sum are carried scaled by 2^B, with
 B = ceil(t_0^2 - s^2 - 1000) set at a restart whose start lies before the
-peak and more than 2^-1000 below the column's largest term (else 0), so that G never starts
+peak, short of the clamp, and more than 2^-1000 below the column's largest
+term (else 0), so that G never starts
 subnormal; a column with s^2 >= 1100 is not run (its sum is below 2^-1100 of
 its weights); a row whose terms all lie below 2^-60 of the column's largest
 is skipped, with G set to 0, when that holds on every lane of the wave.  This restates that control flow
@@ -36,8 +38,10 @@ SCALE_T2 = 1000.0
 
 
 def exp2_neg(a):
-    return np.exp2(-np.maximum(np.asarray(a, dtype=np.float64), -1100.0)) if np.ndim(a) else \
-        float(np.exp2(-max(a, -1100.0)))
+    """2^-a with the exponent held at -1100 and above, as the kernel's does
+    (so a large a gives 0)."""
+    return np.exp2(np.maximum(-np.asarray(a, dtype=np.float64), -1100.0)) if np.ndim(a) else \
+        float(np.exp2(max(-a, -1100.0)))
 
 
 def min_t2(t_first, t_last):
@@ -83,10 +87,13 @@ def columns_by_recurrence(t_first, d, rows, w, underflow_restart=True, skip=True
                 tc = max(t0[j], -lim)
                 R[j] = exp2_neg(2.0 * tc * d + d * d)
                 D = tc * tc - s2_run[j]
-                Bn = int(np.ceil(D - SCALE_T2)) if (scale and t0[j] < 0.0 and D > SCALE_T2) else 0
+                # scaled only before the peak and short of the clamp; a
+                # clamped start keeps B = 0 and its G = 2^-1600 is 0
+                deep = scale and -lim < t0[j] < 0.0 and D > SCALE_T2
+                Bn = int(np.ceil(D - SCALE_T2)) if deep else 0
                 total[j] = np.ldexp(total[j], Bn - B[j])
                 B[j] = Bn
-                G[j] = exp2_neg(D - Bn) if t0[j] > -lim else 0.0
+                G[j] = exp2_neg(D - Bn)
         for i in range(ROW):
             total += w[k * ROW + i] * G
             G *= R
