@@ -339,11 +339,45 @@ __device__ __forceinline__ void bg_point(double r, double qw, const Quad& q, con
     }
 }
 
+// The Hernquist background term at a quarter of its value, one instruction
+// cheaper than bg_point: the Newton step for 1/sqrt(rg^2) in the product form
+// Y = y (3 - rg^2 y^2) gives 2 / rg (three instructions besides the estimate,
+// against four for y + y (1 - rg^2 y^2) / 2), so rg^2 Y + 2 r0 = 2 (rg + r0),
+// its reciprocal X = 1 / (2 (rg + r0)) and qw Y X^3 = qw / (4 rg (rg + r0)^3).
+// Every factor is exact (powers of two), the rounding that of bg_point's
+// (the product form adds under 1e-16 to the step's 4e-15).  The caller
+// multiplies the sum by 4 once.
+__device__ __forceinline__ void bg_point_quarter(double r, double qw, const Quad& q, double r0x2, double& bg)
+{
+    const double rg2 = quad_at(q, r);
+    const double y = __builtin_amdgcn_rsq(rg2);
+    const double Y = y * fma(-rg2, y * y, 3.0);
+    const double X = rcp_nr(fma(rg2, Y, r0x2));
+    bg = fma(qw * Y, X * X * X, bg);
+}
+
+// Mixed precision's background term: the Hernquist profile at a quarter of
+// its value (bg_point_quarter), the other profiles as bg_point; a sum of them
+// is restored by kMixedBgScale.
+template <int BG>
+__device__ __forceinline__ void bg_point_mixed(double r, double qw, const Quad& q, const EvalConst& K, double& bg)
+{
+    if constexpr (BG == kBgHernquist)
+        bg_point_quarter(r, qw, q, 2.0 * K.r0, bg);
+    else
+        bg_point<BG>(r, qw, q, K, bg);
+}
+template <int BG>
+constexpr double kMixedBgScale = BG == kBgHernquist ? 4.0 : 1.0;
+
 template <int S, int BG, bool MIXED, bool SHIFTED = false>
 __device__ __forceinline__ void point_terms(double r, double qw, const Sight<S>& v,
                                             const EvalConst& K, double& bg, double (&st)[kMaxStreams])
 {
-    bg_point<BG>(r, qw, v.bg, K, bg);
+    if constexpr (MIXED && SHIFTED)
+        bg_point_mixed<BG>(r, qw, v.bg, K, bg);   // the caller restores the sum by kMixedBgScale
+    else
+        bg_point<BG>(r, qw, v.bg, K, bg);
 #pragma unroll
     for (int s = 0; s < S; s++) {
         if constexpr (MIXED && SHIFTED) {
@@ -467,14 +501,14 @@ __device__ __forceinline__ void row_by_recurrence(const double2* row, int n, dou
 #pragma unroll 2
         for (int i = 0; i < n; i++) {
             const double2 e = row[i];
-            bg_point<BG>(e.x, e.y, v.bg, K, acc[0]);
+            bg_point_mixed<BG>(e.x, e.y, v.bg, K, acc[0]);
         }
     } else {
         // (stream weight, background weight) entries; node i sits at r0 + i h
         // and only every Q-th node carries a background weight
         double rb = r0;
         for (int i = 0; i < n; i += Q) {
-            bg_point<BG>(rb, row[i].y, v.bg, K, acc[0]);
+            bg_point_mixed<BG>(rb, row[i].y, v.bg, K, acc[0]);
             rb += Q * h;
         }
     }
@@ -574,7 +608,7 @@ __device__ __forceinline__ void integral_block_recurrence(const RegionLaunch& it
     if (valid) {
         const double wnu = it.col[3 * C + cc];
         const Sight<S> again = make_sight<S, BG, true>(it.col[cc], it.col[C + cc], it.col[2 * C + cc], K);
-        out[0] = acc[0] * wnu;
+        out[0] = acc[0] * wnu * kMixedBgScale<BG>;
 #pragma unroll
         for (int s = 0; s < S; s++)
             out[1 + s] = ldexp(acc[1 + s], -B[s]) * wnu * K.on[s] * exp2_neg(again.st[s].e + s2[s]);
@@ -612,7 +646,7 @@ __device__ __forceinline__ void integral_block_split(const RegionLaunch& it, lon
         const Sight<S> sight = make_sight<S, BG, true>(it.col[cc], it.col[C + cc], it.col[2 * C + cc], K);
         const ConstPtr bq = (ConstPtr)(it.bg_rq);
 #pragma unroll 2
-        for (int i = 0; i < it.bg_count; i++) bg_point<BG>(bq[2 * i], bq[2 * i + 1], sight.bg, K, acc[0]);
+        for (int i = 0; i < it.bg_count; i++) bg_point_mixed<BG>(bq[2 * i], bq[2 * i + 1], sight.bg, K, acc[0]);
 #pragma unroll
         for (int s = 0; s < S; s++) {
             const int R = it.st_rows[s];
@@ -695,7 +729,7 @@ __device__ __forceinline__ void integral_block_split(const RegionLaunch& it, lon
     for (int c = 0; c <= S; c++) out[c] = 0.0;
     if (valid) {
         const double wnu = it.col[3 * C + cc];
-        out[0] = acc[0] * wnu;
+        out[0] = acc[0] * wnu * kMixedBgScale<BG>;
 #pragma unroll
         for (int s = 0; s < S; s++) out[1 + s] = acc[1 + s] * wnu * K.on[s];
     }
@@ -786,7 +820,9 @@ __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
             }
         }
         if (valid) {
-            out[0] = DD{ (acc[0] + cmp[0]) * wnu, 0.0 };
+            // mixed: point_terms summed the background by bg_point_mixed
+            constexpr double bg_scale = MIXED ? kMixedBgScale<BG> : 1.0;
+            out[0] = DD{ (acc[0] + cmp[0]) * wnu * bg_scale, 0.0 };
 #pragma unroll
             for (int s = 0; s < S; s++) {
                 double v = (acc[1 + s] + cmp[1 + s]) * wnu * K.on[s];
@@ -1003,15 +1039,15 @@ __device__ __forceinline__ void star_densities(const double* __restrict__ star, 
         if (bg_count > 0) {
             const ConstPtr bq = (ConstPtr)bg_tab;
 #pragma unroll 2
-            for (int m = 0; m < bg_count; m++) bg_point<BG>(rstar * bq[2 * m], bq[2 * m + 1], sight.bg, K, b);
+            for (int m = 0; m < bg_count; m++) bg_point_mixed<BG>(rstar * bq[2 * m], bq[2 * m + 1], sight.bg, K, b);
             b *= rs3;
         } else {
             for (int j = 0; j < n; j++) {
                 const double r = rstar * ratio[j];
-                bg_point<BG>(r, kw[j] * (r * r * r), sight.bg, K, b);
+                bg_point_mixed<BG>(r, kw[j] * (r * r * r), sight.bg, K, b);
             }
         }
-        bg = b;
+        bg = b * kMixedBgScale<BG>;
     } else {
         for (int j = 0; j < n; j++) {
             const double r = rstar * ratio[j];
