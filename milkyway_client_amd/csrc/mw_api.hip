@@ -109,6 +109,7 @@ int build_context(mw_ctx* c, const std::vector<RegionGeom>& geoms, int n_conv, i
             gmax = std::max(gmax, gm);
         }
         rd->g_span = gmax - gmin;
+        rd->r_min = rmin;
         rd->r_span = rd->r_max - rmin;
         for (long col = 0; col < t.n_cols; col++) {
             const double ux = t.col[col];
@@ -116,6 +117,8 @@ int build_context(mw_ctx* c, const std::vector<RegionGeom>& geoms, int n_conv, i
         }
         tabulate_bg_width(*rd, t.col.data(), t.col.data() + t.n_cols, t.col.data() + 2 * t.n_cols,
                           g.mu_steps, g.nu_steps);
+        tabulate_sights(*rd, t.col.data(), t.col.data() + t.n_cols, t.col.data() + 2 * t.n_cols,
+                        g.mu_steps, g.nu_steps);
         const char* bg_env = std::getenv("MW_HIP_BG_GRID");
         rd->bg_grid = !(bg_env && std::strcmp(bg_env, "0") == 0);
         rd->h_rq = std::move(t.rq);
@@ -225,6 +228,8 @@ int run_plan(mw_ctx* c, Plan& P)
             L.block_begin = g.n_blocks;          // block index within the group's launch
             L.block_count = rd.blocks;
             L.n_streams = nS;
+            L.r_lo = rd.r_min;
+            L.r_hi = rd.r_max;
             L.grid_step = layout > 0 ? recurrence_step(level, P.K[pr.slot]) : 0.0;
             L.grid_lo = tr.lo;
             L.bg_stride = layout_bg_stride(layout);

@@ -143,6 +143,8 @@ int BatchRunner::plan(const std::vector<Slot>& slots, int precision)
             L.out_index = row++;
             L.block_count = rd->blocks;
             L.n_streams = slots[s].k.n_streams;
+            L.r_lo = rd->r_min;
+            L.r_hi = rd->r_max;
             L.grid_step = layout > 0 ? recurrence_step(level, slots[s].k) : 0.0;
             L.grid_lo = tr.lo;
             L.bg_stride = layout_bg_stride(layout);

@@ -290,7 +290,7 @@ struct RegionDev {
     // the Galactic centre is 8.5 kpc times that for q = 1).
     std::vector<double> h_rq;
     std::vector<RStep> h_rstep;
-    double r_max = 0.0, r_span = 0.0, g_span = 0.0, min_sin_gc = 1.0;
+    double r_min = 0.0, r_max = 0.0, r_span = 0.0, g_span = 0.0, min_sin_gc = 1.0;
     // Smallest width of the background's peak along the region's sights
     // (kpc), tabulated at q = 2^(j / 4 - kWidthQ0), from a grid of sights
     // that includes the region's edges (see bg_width).
@@ -299,6 +299,12 @@ struct RegionDev {
     // off the real axis of the magnitude variable along the region's sights,
     // at the same q (see bg_theta).
     std::vector<float> bg_theta_q;
+    // Sight directions sampled over the region (up to 32 x 8, its first and
+    // last rows and columns included; 3 doubles each) and the cosine of the
+    // largest angle between a sight of the region and its nearest sample
+    // (see streams_peak_inside).
+    std::vector<double> sample_u;
+    double sample_cos = 1.0;
     // Layout 4 allowed (off when the environment has MW_HIP_BG_GRID=0 as the
     // region is built: the background then stays on the distance grid).
     bool bg_grid = true;
@@ -358,6 +364,14 @@ int bg_grid_level(const RegionDev& rd, double q);
 constexpr double kFoldQMin = 0.3, kFoldQMax = 1.6, kFoldR0Min = 1.0, kFoldR0Max = 30.0;
 constexpr double kFoldSigmaMin = 0.1;
 bool in_measured_domain(double q, double r0);
+// Sample the region's sight directions (RegionDev::sample_u, sample_cos).
+void tabulate_sights(RegionDev& rd, const double* ux, const double* uy, const double* uz, int mu_steps,
+                     int nu_steps);
+// Whether every switched-on stream's Gaussian peaks inside the region's
+// distances along the sights that carry its integral over the region (the
+// resampled grids resolve it there; a stream that passes outside the region
+// leaves only steep tails, which they do not).
+bool streams_peak_inside(const RegionDev& rd, const EvalConst& k);
 
 // Resampled table index to use for region rd under the model k, or -1 for
 // the exact tables (resampling off, a background mode it does not cover, a

@@ -89,6 +89,9 @@ struct RegionLaunch {
     int out_index;             // index into the per-region result rows
     int block_begin, block_count;
     int n_streams;             // streams of the slot's model (batched kernels)
+    double r_lo, r_hi;         // exact tables: the smallest and largest r of the region's points
+                               // (the mixed kernels shift each stream's exponent by its smallest
+                               // t^2 over them)
     double grid_step;          // > 0: rows of nodes uniform in distance at this spacing, in the
                                // recurrence layout (mixed kernels take the stream Gaussians by
                                // recurrence); node m of the region sits at grid_lo + m grid_step

@@ -370,9 +370,15 @@ __device__ __forceinline__ void bg_point_mixed(double r, double qw, const Quad& 
 template <int BG>
 constexpr double kMixedBgScale = BG == kBgHernquist ? 4.0 : 1.0;
 
+// SHIFTED (mixed exact tables): stream s's exponent is t^2 - s2[s], its
+// smallest value over the region's points taken out (the caller scales each
+// sum by 2^-(e + s2)), so that the FP32 2^x of the terms that carry the sum
+// stays far from underflow even where the sight's closest approach to the
+// stream lies outside the region's distances.
 template <int S, int BG, bool MIXED, bool SHIFTED = false>
 __device__ __forceinline__ void point_terms(double r, double qw, const Sight<S>& v,
-                                            const EvalConst& K, double& bg, double (&st)[kMaxStreams])
+                                            const EvalConst& K, double& bg, double (&st)[kMaxStreams],
+                                            const double* s2 = nullptr)
 {
     if constexpr (MIXED && SHIFTED)
         bg_point_mixed<BG>(r, qw, v.bg, K, bg);   // the caller restores the sum by kMixedBgScale
@@ -382,7 +388,7 @@ __device__ __forceinline__ void point_terms(double r, double qw, const Sight<S>&
     for (int s = 0; s < S; s++) {
         if constexpr (MIXED && SHIFTED) {
             const double t = fma(v.st[s].p, r, -v.st[s].m);
-            const float e = __builtin_amdgcn_exp2f(-(float)(t * t));
+            const float e = __builtin_amdgcn_exp2f(-(float)fma(t, t, -s2[s]));
             st[s] = fma(qw, (double)e, st[s]);
             continue;
         }
@@ -784,6 +790,14 @@ __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
         double acc[1 + S], cmp[1 + S];
 #pragma unroll
         for (int c = 0; c <= S; c++) acc[c] = cmp[c] = 0.0;
+        // mixed: each stream's smallest t^2 over the region's points (0 where
+        // the sight's closest approach to the stream lies among them)
+        double s2[S > 0 ? S : 1];
+#pragma unroll
+        for (int s = 0; s < S; s++)
+            s2[s] = MIXED ? min_t2(fma(sight.st[s].p, it.r_lo, -sight.st[s].m),
+                                   fma(sight.st[s].p, it.r_hi, -sight.st[s].m))
+                          : 0.0;
 
         const int rt = kTileD2 / n;   // r-steps per LDS tile (host guarantees n <= kTileD2)
         const double2* rq = reinterpret_cast<const double2*>(it.rq);
@@ -800,7 +814,7 @@ __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
 #pragma unroll UNROLL
                 for (int i = 0; i < n; i++) {
                     const double2 p = row[i];
-                    point_terms<S, BG, MIXED, MIXED>(p.x, p.y, sight, K, bg, st);
+                    point_terms<S, BG, MIXED, MIXED>(p.x, p.y, sight, K, bg, st, s2);
                 }
                 const RStep& rs = it.rstep[k];
                 double f = rs.reff_xr_rp3 * rs.irv;
@@ -826,7 +840,7 @@ __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
 #pragma unroll
             for (int s = 0; s < S; s++) {
                 double v = (acc[1 + s] + cmp[1 + s]) * wnu * K.on[s];
-                if constexpr (MIXED) v *= exp2_neg(sight.st[s].e);   // the minimum taken out per point
+                if constexpr (MIXED) v *= exp2_neg(sight.st[s].e + s2[s]);   // the shift taken out per point
                 out[1 + s] = DD{ v, 0.0 };
             }
         }

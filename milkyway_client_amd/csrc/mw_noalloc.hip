@@ -126,6 +126,8 @@ extern "C" int mw_ctx_eval_noalloc(mw_ctx* c, const double* params, int precisio
         L.block_begin = iblocks;
         L.block_count = rd.blocks;
         L.n_streams = S;
+        L.r_lo = rd.r_min;
+        L.r_hi = rd.r_max;
         ired[g] = ReduceItem{ iblocks, rd.blocks, g, 0 };
         iblocks += rd.blocks;
     }

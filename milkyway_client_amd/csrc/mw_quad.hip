@@ -169,11 +169,99 @@ bool in_measured_domain(double q, double r0)
     return q >= kFoldQMin && q <= kFoldQMax && r0 >= kFoldR0Min && r0 <= kFoldR0Max;
 }
 
+void tabulate_sights(RegionDev& rd, const double* ux, const double* uy, const double* uz, int mu_steps,
+                     int nu_steps)
+{
+    std::vector<long> rows, cols;
+    const int nm = std::min(mu_steps, 32), nn = std::min(nu_steps, 8);
+    for (int i = 0; i < nm; i++) rows.push_back(nm > 1 ? (long)i * (mu_steps - 1) / (nm - 1) : 0);
+    for (int j = 0; j < nn; j++) cols.push_back(nn > 1 ? (long)j * (nu_steps - 1) / (nn - 1) : 0);
+    rd.sample_u.clear();
+    for (long r : rows)
+        for (long c : cols) {
+            const long i = r * nu_steps + c;
+            rd.sample_u.insert(rd.sample_u.end(), { ux[i], uy[i], uz[i] });
+        }
+    // A sight lies within half a sample cell's diagonal of one of its corners.
+    auto u = [&](size_t a, size_t b, int j) { return rd.sample_u[3 * (a * cols.size() + b) + j]; };
+    double worst = 1.0;
+    for (size_t a = 0; a + 1 < rows.size() || (a == 0 && rows.size() == 1); a++)
+        for (size_t b = 0; b + 1 < cols.size() || (b == 0 && cols.size() == 1); b++) {
+            const size_t a1 = std::min(a + 1, rows.size() - 1), b1 = std::min(b + 1, cols.size() - 1);
+            for (int d = 0; d < 2; d++) {
+                const size_t bb = d ? b1 : b, bc = d ? b : b1;
+                const double cd = u(a, bb, 0) * u(a1, bc, 0) + u(a, bb, 1) * u(a1, bc, 1) + u(a, bb, 2) * u(a1, bc, 2);
+                worst = std::min(worst, std::cos(0.5 * std::acos(std::max(-1.0, std::min(1.0, cd)))));
+            }
+        }
+    rd.sample_cos = worst;
+}
+
+bool streams_peak_inside(const RegionDev& rd, const EvalConst& k)
+{
+    const size_t N = rd.sample_u.size() / 3;
+    if (N == 0) return false;
+    const double r_lo = rd.r_min, r_hi = rd.r_max;
+    for (int s = 0; s < k.n_streams; s++) {
+        if (k.on[s] == 0.0 || !(k.inv_2s2[s] > 0.0)) continue;
+        // The centre's own sight reaches the axis at the centre: a centre
+        // inside the region (its distance in range, its direction within a
+        // sample cell of the region's sights) puts the peak inside.
+        const double cp[3] = { k.sc[s][0] + kSunR0, k.sc[s][1], k.sc[s][2] };
+        const double rc = std::sqrt(cp[0] * cp[0] + cp[1] * cp[1] + cp[2] * cp[2]);
+        if (rc >= r_lo && rc <= r_hi) {
+            const double near = rd.sample_cos * rc;
+            bool in_cell = false;
+            for (size_t i = 0; i < N && !in_cell; i++) {
+                const double* u = &rd.sample_u[3 * i];
+                in_cell = u[0] * cp[0] + u[1] * cp[1] + u[2] * cp[2] >= near;
+            }
+            if (in_cell) continue;
+        }
+        // Otherwise the sampled sights: the exponent's smallest value over
+        // the region's distances (a = e + t^2 at the nearest end, t = p r - m)
+        // over all of them, and over those whose closest approach to the
+        // axis lies inside the distances.  The latter must come within a
+        // factor 2 of the former: the integral is then carried by sights
+        // whose peak the grid resolves.
+        double a_all = INFINITY, a_in = INFINITY;
+        for (size_t i = 0; i < N; i++) {
+            const double* u = &rd.sample_u[3 * i];
+            const double d = u[0] * k.sa[s][0] + u[1] * k.sa[s][1] + u[2] * k.sa[s][2];
+            const double w0 = (u[0] - d * k.sa[s][0]) * k.ks[s];
+            const double w1 = (u[1] - d * k.sa[s][1]) * k.ks[s];
+            const double w2 = (u[2] - d * k.sa[s][2]) * k.ks[s];
+            const double ww = w0 * w0 + w1 * w1 + w2 * w2;
+            const double c0 = k.cperp[s][0], c1 = k.cperp[s][1], c2 = k.cperp[s][2];
+            double a;
+            bool inside;
+            if (ww > 1e-200) {
+                const double p = std::sqrt(ww);
+                const double m = (w0 * c0 + w1 * c1 + w2 * c2) / p;
+                const double e0 = c0 - m * w0 / p, e1 = c1 - m * w1 / p, e2 = c2 - m * w2 / p;
+                const double t_lo = p * r_lo - m, t_hi = p * r_hi - m;
+                inside = t_lo <= 0.0 && t_hi >= 0.0;
+                const double t = inside ? 0.0 : std::min(std::fabs(t_lo), std::fabs(t_hi));
+                a = e0 * e0 + e1 * e1 + e2 * e2 + t * t;
+            } else {
+                // sight parallel to the axis: the same value at every distance
+                inside = true;
+                a = c0 * c0 + c1 * c1 + c2 * c2;
+            }
+            a_all = std::min(a_all, a);
+            if (inside) a_in = std::min(a_in, a);
+        }
+        if (!(a_in <= a_all + 1.0)) return false;
+    }
+    return true;
+}
+
 int select_level(const RegionDev& rd, const EvalConst& k, bool resample)
 {
     if (!resample || rd.h_rq.empty()) return -1;
     if (k.bg_mode != kBgHernquist && k.bg_mode != kBgHernquistAux) return -1;
     if (!in_measured_domain(k.q, k.r0)) return -1;
+    if (!streams_peak_inside(rd, k)) return -1;
     double sigma = INFINITY;
     for (int s = 0; s < k.n_streams; s++) {
         if (k.on[s] == 0.0 || !(k.inv_2s2[s] > 0.0)) continue;

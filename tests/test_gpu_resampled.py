@@ -115,6 +115,52 @@ def test_reference_bounds_against_oracle(q, r0, sigma):
     assert e["I_bg"] <= INT_REL_TOL and e["I_st"] <= INT_REL_TOL
 
 
+@pytest.mark.parametrize("r", [130.0, 140.0])
+def test_stream_far_outside_the_wedge(r):
+    """A 1 kpc stream whose axis passes far outside the wedge (centre 130 or
+    140 kpc out): every sight's terms lie 2^-480 to 2^-1000 below a sight
+    through the stream, and the stream integral is 2^-481 or 2^-918, normal
+    numbers in the oracle's FP64.  The
+    integral still normalises the stream's per-star density, so it must
+    match to the bars in every precision: the exact paths take each sight's
+    minimum over the region's distances out of its exponents.  The resampled
+    grids cannot resolve such steep tails (their fold would extrapolate the
+    integrand past the region's ends), so a resampled request keeps the
+    exact tables here: its integrals are the mixed path's bitwise."""
+    p, stars = synth.workload(302, LONG_R, 1)
+    st = p.streams[0].parameters
+    st[1], st[2], st[3], st[4] = r, 1.047, 1.5708, 1.0
+    ref = O.evaluate_full(ParamsHolder(p), StarsHolder(stars))
+    assert 0.0 < ref.stream_integrals[0] < 1e-140
+    ph, sh = ParamsHolder(p), StarsHolder(stars)
+    got = {}
+    with sep.Context(ph, sh, timing=True) as ctx:
+        for prec, int_tol, lnp_tol in ((sep.PREC_FP64, 1e-12, 1e-10), (sep.PREC_MIXED, INT_REL_TOL, LNP_TOL),
+                                       (sep.PREC_RESAMPLED, INT_REL_TOL, LNP_TOL)):
+            g = got[prec] = ctx.eval(p.parameter_vector(), precision=prec, per_star=True)[0]
+            tm = ctx.timing()
+            e = dict(I_bg=rel(g.background_integral, ref.background_integral),
+                     I_st=rel(g.stream_integrals, ref.stream_integrals),
+                     lnP=math.log(10) * np.max(np.abs(g.log10_p - ref.log10_p)))
+            print(r, prec, {k: f"{v:.2e}" for k, v in e.items()}, "I_st", g.stream_integrals[0],
+                  "points", tm.integral_points)
+            assert e["I_st"] <= int_tol and e["I_bg"] <= int_tol and e["lnP"] <= lnp_tol
+    mixed, resampled = got[sep.PREC_MIXED], got[sep.PREC_RESAMPLED]
+    assert resampled.background_integral == mixed.background_integral
+    assert np.array_equal(resampled.stream_integrals, mixed.stream_integrals)
+
+
+def test_stream_crossing_the_wedge_keeps_the_grid():
+    """The same stream moved inside the wedge (centre 25 kpc out): the
+    resampled grids are used, and the result is within the bars."""
+    p, stars = synth.workload(302, LONG_R, 1)
+    st = p.streams[0].parameters
+    st[1], st[2], st[3], st[4] = 25.0, 1.047, 1.5708, 1.0
+    e, grew = errors_vs_oracle(p, stars)
+    assert grew, "the resampled grid was not used"
+    assert e["lnP"] <= LNP_TOL and e["I_bg"] <= INT_REL_TOL and e["I_st"] <= INT_REL_TOL
+
+
 @pytest.mark.parametrize("field,value", [("sigma", 0.05), ("r0", 0.5), ("r0", 40.0), ("q", 0.2),
                                          ("q", 2.0)])
 def test_outside_the_measured_domain_keeps_the_exact_tables(field, value):
