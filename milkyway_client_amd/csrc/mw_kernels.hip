@@ -426,23 +426,20 @@ __device__ __forceinline__ void point_terms(double r, double qw, const Sight<S>&
 // scaled by 2^-(e + s^2) at the end.  Without the shift, a sight whose peak
 // lies beyond the grid would start its on-grid terms below 2^-1000, in
 // subnormal numbers or at 0.
-// A restart can still fall on a node far below that largest term: a wave
-// runs the union of its lanes' rows, so a lane may start rows before its own
-// terms matter, and a row of 32 nodes spans up to 16 in t.  A G started in
-// subnormal numbers carries its rounding (up to 100 %) to every later term,
-// the peak's included, until the next restart.  So G and the sight's running
-// sum are carried scaled by 2^B, B a whole number >= 0 set at each restart:
-// B = ceil(t_0^2 - s^2 - 1000) when the start lies before the peak, short of
-// the clamp below, and more than 2^-1000 below the largest term, else 0.  G then starts in [2^-1000, 2^-999), with full
-// precision; the scaled G stays below 2^B <= 2^600 and the scaled sum below
-// 2^610, and rescaling by a power of two is exact.  The sum is unscaled
-// (2^-B) at the end.  The caller guarantees d <= 1/2.  On restart t_0 is
-// clamped at -sqrt(s^2 + 1600), so R stays below 2^(sqrt(s^2 + 1600)), and a
-// clamped start gives G = 2^-1600 = 0, which stays 0 until the next restart: the terms
-// of a row that starts there lie below 2^-((sqrt(s^2 + 1600) - 16)^2 - s^2),
-// 2^-190 or less, of the largest.  A sight whose s^2 reaches 1100 has every
-// on-grid term below 2^-1100 of its peak, so its final factor 2^-(e + s^2)
-// is 0: its recurrence is not run (G = R = 0).
+// A restart can still fall on a node far below that largest term: a wave runs the union of its
+// lanes' rows, so a lane may start rows before its own terms matter, and a row of 32 nodes spans up
+// to 16 in t.  A G started in subnormal numbers carries its rounding (up to 100 %) to every later
+// term, the peak's included, until the next restart.  So G and the sight's running sum are carried
+// scaled by 2^B, B a whole number >= 0 set at each restart: B = ceil(t_0^2 - s^2 - 1000) when the
+// start lies before the peak, short of the clamp below, and more than 2^-1000 below the largest
+// term, else 0.  G then starts in [2^-1000, 2^-999), with full precision; the scaled G stays below
+// 2^B <= 2^600 and the scaled sum below 2^610, and rescaling by a power of two is exact.  The sum
+// is unscaled (2^-B) at the end.  The caller guarantees d <= 1/2.  On restart t_0 is clamped at
+// -sqrt(s^2 + 1600), so R stays below 2^(sqrt(s^2 + 1600)), and a clamped start gives
+// G = 2^-1600, that is 0, which stays 0 until the next restart: the terms of a row that starts there lie below
+// 2^-((sqrt(s^2 + 1600) - 16)^2 - s^2), 2^-190 or less, of the largest.  A sight whose s^2 reaches
+// 1100 has every on-grid term below 2^-1100 of its peak, so its final factor 2^-(e + s^2) is 0: its
+// recurrence is not run (G = R = 0).
 constexpr int kRestartRows = 8;
 constexpr double kRestartClampT2 = 1600.0;
 constexpr double kGoneT2 = 1100.0;
