@@ -521,6 +521,30 @@ def test_batch_equals_single(wedges, graph):
             assert all(same(g, r) for g, r in zip(got, singles))
 
 
+@pytest.mark.parametrize("precision", [sep.PREC_MIXED, sep.PREC_RESAMPLED])
+def test_batch_with_a_stream_outside_the_wedge(precision):
+    """A graph-replayed batch whose middle vector puts its stream 140 kpc
+    out, outside the wedge (the resampled plan keeps the exact tables for
+    that vector alone): every result equals its single evaluation bitwise,
+    on two replays, and the far vector's stream integral (8e-247) matches
+    the oracle."""
+    p, stars = synth.workload(302, LONG_R, 1)
+    vecs = synth.parameter_vectors(302, p, 3)
+    vecs[1, 4:8] = (140.0, 1.047, 1.5708, 1.0)      # r, theta, phi, sigma of the stream
+    ph, sh = ParamsHolder(p), StarsHolder(stars)
+    with sep.Context(ph, sh) as ctx:
+        singles = [ctx.eval(v, precision=precision)[0] for v in vecs]
+        with sep.Batch([(ctx, 3)], precision=precision, graph=True) as b:
+            for _ in range(2):
+                got = b.run([vecs])
+                assert all(same(g, r) for g, r in zip(got, singles))
+    p.set_parameter_vector(vecs[1])
+    ref = O.integrals(ParamsHolder(p), StarsHolder(stars))
+    print("far stream integral", singles[1].stream_integrals[0], "oracle", ref.stream_integrals()[0])
+    assert 0.0 < ref.stream_integrals()[0] < 1e-200
+    assert rel(singles[1].stream_integrals, ref.stream_integrals()) <= INT_REL_TOL
+
+
 @pytest.mark.parametrize("graph", [False, True])
 def test_batch_mixing_split_and_strided_layouts(graph, monkeypatch):
     """One batch over a wedge in the split layout and a wedge whose
