@@ -743,6 +743,56 @@ __device__ __forceinline__ void integral_block_split(const RegionLaunch& it, lon
 // Integral: one logical block = 256 consecutive columns of one region of one
 // evaluation slot; the block's double-double partials go to prow.
 // ---------------------------------------------------------------------------
+// Mixed precision on the exact tables: G consecutive convolution points of
+// one r-step (G = 0: the m < kMaxGroup points left at a row's end).  Each
+// point's background term goes to bg in FP64 (bg_point_mixed); each stream
+// term qw 2^-(t^2 - s2) is formed from the FP64 exponent by v_exp_f32 and
+// summed over the group in FP32, with qw rounded to FP32 (qw32), then the
+// group's sum is added to st[s] in FP64.  Per stream and point that is one
+// FP32 FMA, against a conversion to FP64 and an FP64 FMA, plus one
+// conversion and one FP64 add per group.
+// Error of a group's sum: each of the G products and G - 1 additions rounds
+// to nearest (relative 2^-24, unbiased).  An addition drops, always
+// downwards, a term under half an ulp of its running sum, that is under
+// 2^-24 of the group's largest term: a bias below (G - 1) 2^-24 of the
+// group, and only where one group's terms span 24 octaves.  Neighbouring
+// convolution points' terms differ by a factor 2^(2 |t| dt), dt the step of
+// t between them, so that happens only far out in the Gaussian's tail, whose
+// groups are negligible against the sight's peak.  qw32 carries a relative
+// error below 2^-24 per weight, the same in every column and parameter
+// vector and of random sign across the thousands of points a Gaussian spans.
+// The group size is the same in every kernel variant, so that the tuner's
+// choice does not change a result's bits.
+constexpr int kMaxGroup = 8;
+constexpr int kStreamGroup = kMaxGroup;
+
+template <int S, int BG, int G>
+__device__ __forceinline__ void stream_group(const double2* row, const float* row32, const Sight<S>& v,
+                                             const EvalConst& K, const double (&s2)[S > 0 ? S : 1],
+                                             double& bg, double (&st)[kMaxStreams], int m = G)
+{
+    static_assert(G <= kMaxGroup, "group larger than kMaxGroup");
+    float g[S > 0 ? S : 1];
+#pragma unroll
+    for (int s = 0; s < S; s++) g[s] = 0.0f;
+#pragma unroll
+    for (int u = 0; u < (G > 0 ? G : kMaxGroup); u++) {
+        if (G == 0 && u >= m) break;
+        const double2 p = row[u];
+        const float qw32 = row32[u];
+        bg_point_mixed<BG>(p.x, p.y, v.bg, K, bg);
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            const double t = fma(v.st[s].p, p.x, -v.st[s].m);
+            const float e = __builtin_amdgcn_exp2f(-(float)fma(t, t, -s2[s]));
+            g[s] = __builtin_fmaf(qw32, e, g[s]);
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < S; s++) st[s] += (double)g[s];
+}
+
+// ---------------------------------------------------------------------------
 // REC: the mixed kernels' recurrence and split layouts compiled in (items
 // with grid_step > 0); without them the kernel holds the exact tables' loop
 // alone, and its register allocation is that loop's.
@@ -750,7 +800,7 @@ template <int S, int BG, bool MIXED, int UNROLL, bool REC = true>
 __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
                                                const EvalConst* __restrict__ consts,
                                                double* __restrict__ prow, double2* tile,
-                                               DD (*red)[kIntComps])
+                                               float* tile32, DD (*red)[kIntComps])
 {
     if constexpr (MIXED && REC) {
         if (it.grid_step > 0.0) {
@@ -801,17 +851,29 @@ __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
         for (int k0 = 0; k0 < R; k0 += rt) {
             const int kt = (R - k0 < rt) ? R - k0 : rt;
             __syncthreads();
-            for (int e = threadIdx.x; e < kt * n; e += kBlock) tile[e] = rq[(long)k0 * n + e];
+            for (int e = threadIdx.x; e < kt * n; e += kBlock) {
+                const double2 v = rq[(long)k0 * n + e];
+                tile[e] = v;
+                if constexpr (MIXED) tile32[e] = (float)v.y;
+            }
             __syncthreads();
             for (int kk = 0; kk < kt; kk++) {
                 const int k = k0 + kk;
                 double bg = 0.0;
                 double st[kMaxStreams] = { 0.0, 0.0, 0.0, 0.0 };
                 const double2* row = tile + kk * n;
+                if constexpr (MIXED) {
+                    const float* row32 = tile32 + kk * n;
+                    int i = 0;
+                    for (; i + kStreamGroup <= n; i += kStreamGroup)
+                        stream_group<S, BG, kStreamGroup>(row + i, row32 + i, sight, K, s2, bg, st);
+                    if (i < n) stream_group<S, BG, 0>(row + i, row32 + i, sight, K, s2, bg, st, n - i);
+                } else {
 #pragma unroll UNROLL
-                for (int i = 0; i < n; i++) {
-                    const double2 p = row[i];
-                    point_terms<S, BG, MIXED, MIXED>(p.x, p.y, sight, K, bg, st, s2);
+                    for (int i = 0; i < n; i++) {
+                        const double2 p = row[i];
+                        point_terms<S, BG, false>(p.x, p.y, sight, K, bg, st);
+                    }
                 }
                 const RStep& rs = it.rstep[k];
                 double f = rs.reff_xr_rp3 * rs.irv;
@@ -861,25 +923,26 @@ __global__ __launch_bounds__(kBlock) void integral_kernel(const RegionLaunch* __
                                                           double* __restrict__ partials)
 {
     __shared__ double2 tile[kTileD2];
+    __shared__ float tile32[MIXED ? kTileD2 : 1];
     __shared__ DD red[kBlock / 64][kIntComps];
     const int b = blockIdx.x;
     const RegionLaunch& it = items[find_item(items, n_items, b)];
     integral_block<S, BG, MIXED, 2, REC>(it, b - it.block_begin, consts,
-                                         partials + (long)b * kIntComps * 2, tile, red);
+                                         partials + (long)b * kIntComps * 2, tile, tile32, red);
 }
 
 template <int BG, bool MIXED, int UNROLL, bool REC>
 __device__ __forceinline__ void integral_block_any(const RegionLaunch& it, long lb,
                                                    const EvalConst* __restrict__ consts,
                                                    double* __restrict__ prow, double2* tile,
-                                                   DD (*red)[kIntComps])
+                                                   float* tile32, DD (*red)[kIntComps])
 {
     switch (it.n_streams) {
-    case 0: integral_block<0, BG, MIXED, UNROLL, REC>(it, lb, consts, prow, tile, red); break;
-    case 1: integral_block<1, BG, MIXED, UNROLL, REC>(it, lb, consts, prow, tile, red); break;
-    case 2: integral_block<2, BG, MIXED, UNROLL, REC>(it, lb, consts, prow, tile, red); break;
-    case 3: integral_block<3, BG, MIXED, UNROLL, REC>(it, lb, consts, prow, tile, red); break;
-    default: integral_block<4, BG, MIXED, UNROLL, REC>(it, lb, consts, prow, tile, red); break;
+    case 0: integral_block<0, BG, MIXED, UNROLL, REC>(it, lb, consts, prow, tile, tile32, red); break;
+    case 1: integral_block<1, BG, MIXED, UNROLL, REC>(it, lb, consts, prow, tile, tile32, red); break;
+    case 2: integral_block<2, BG, MIXED, UNROLL, REC>(it, lb, consts, prow, tile, tile32, red); break;
+    case 3: integral_block<3, BG, MIXED, UNROLL, REC>(it, lb, consts, prow, tile, tile32, red); break;
+    default: integral_block<4, BG, MIXED, UNROLL, REC>(it, lb, consts, prow, tile, tile32, red); break;
     }
 }
 
@@ -897,6 +960,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void integral_persistent(
     const EvalConst* __restrict__ consts, double* __restrict__ partials, unsigned* counter)
 {
     __shared__ double2 tile[kTileD2];
+    __shared__ float tile32[MIXED ? kTileD2 : 1];
     __shared__ DD red[kBlock / 64][kIntComps];
     __shared__ int next;
     for (;;) {
@@ -907,7 +971,7 @@ __global__ __launch_bounds__(kBlock, WAVES) void integral_persistent(
         if (b >= total_blocks) break;
         const RegionLaunch& it = items[find_item(items, n_items, b)];
         integral_block_any<BG, MIXED, UNROLL, REC>(it, b - it.block_begin, consts,
-                                                   partials + (long)b * kIntComps * 2, tile, red);
+                                                   partials + (long)b * kIntComps * 2, tile, tile32, red);
     }
 }
 
