@@ -478,13 +478,14 @@ def accuracy(got, ref) -> dict:
 def tune(groups, vset, precision, args, label):
     pinned = args.variant if precision == sep.PREC_MIXED else args.resampled_variant
     if pinned is not None:
-        return {"variant": pinned[0], "blocks_per_cu": pinned[1], "table": [], "s": 0.0}
+        return {"variant": pinned[0], "blocks_per_cu": pinned[1], "table": [], "s": 0.0,
+                "scratch_bytes": None}
     t0 = time.perf_counter()
     choice = autotune.tune(groups, vset, precision, proxy_vectors=1, reps=1,
                            blocks_per_cu=(2, 4, 5), use_cache=False, log=log)
     choice["s"] = round(time.perf_counter() - t0, 2)
-    log(f"autotuner ({label}): variant {choice['variant']} blocks/CU {choice['blocks_per_cu']} "
-        f"({choice['s']} s)")
+    log(f"autotuner ({label}): variant {choice['variant']} blocks/CU {choice['blocks_per_cu']}, "
+        f"{choice['scratch_bytes']} B of spills per work-item ({choice['s']} s)")
     return choice
 
 
@@ -560,7 +561,10 @@ def bench_batch(args):
                   "prices each stream's exponent at 22 ops and its exp at 20 flop-eq, while the "
                   "kernel spends 2 FP64 operations on the exponent and one v_exp_f32; the counters "
                   "(hardware) are the measure that depends on no model")),
-        "tuning": {"seconds": choice["s"], "table": choice["table"]},
+        "tuning": {"seconds": choice["s"], "table": choice["table"],
+                   "scratch_bytes_of_choice": choice["scratch_bytes"],
+                   "rule": "fastest variant without register spills unless a spilling one is "
+                           f"more than {autotune.SPILL_MARGIN:.0%} faster"},
     }
 
     # ---- the resampled quadrature, fresh vectors every step ---------------

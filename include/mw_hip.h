@@ -229,6 +229,10 @@ int mw_batch_run(mw_batch* batch, const double* params, mw_result* results);
  * (1-16), as chosen by the autotuner. */
 int mw_batch_set_variant(mw_batch* batch, int variant, int blocks_per_cu);
 int mw_batch_variant_count(void);
+/* Scratch (register spill) bytes per work-item of the integral kernels this
+ * batch launches at the given variant (after its first mw_batch_run, which
+ * plans the launch); the autotuner prefers variants that spill nothing. */
+int mw_batch_variant_scratch(const mw_batch* batch, int variant, size_t* bytes);
 int mw_batch_vector_count(const mw_batch* batch);
 /* Timing of the last run: total_ms from the constants' upload to the results'
  * download; integral_ms the persistent integral launch(es) alone (equal to

@@ -4,7 +4,10 @@ For a batch signature (each wedge's r x mu x nu grid, convolution points,
 stream count, star count and vector count, plus the precision) it times
 every kernel variant (inner-loop unroll 2/4 x occupancy target default/4/5
 waves per EU) at several persistent grid sizes (1, 2, 4, 5 or 8 workgroups per
-compute unit), picks the fastest and caches the choice in a JSON file.
+compute unit) and caches its choice in a JSON file: the fastest of the
+variants whose kernels spill no registers to scratch, unless a spilling one
+is more than SPILL_MARGIN faster (spills cost scratch traffic that the
+proxy's timing hardly shows).
 
 Timing runs on a proxy of the batch: the same resident wedges with at most
 ``proxy_vectors`` vectors each, so tuning costs a small fraction of one full
@@ -28,6 +31,7 @@ BLOCKS_PER_CU = (1, 2, 4, 5, 8)
 # (variant v = unroll {2, 4} x occupancy target {compiler's, 4, 5} waves),
 # so that cached choices of an older table are not reused.
 VARIANT_TABLE = 2
+SPILL_MARGIN = 0.01
 
 
 def signature(groups, precision: int) -> str:
@@ -64,6 +68,7 @@ def tune(groups, vectors: Sequence, precision: int, proxy_vectors: int = 1, reps
     proxy = [(ctx, min(n, proxy_vectors)) for ctx, n, _ in groups]
     pvecs = [v[:n] for (_, n), v in zip(proxy, vectors)]
     table: List[Tuple[int, int, float]] = []
+    spills: Dict[int, int] = {}
     with sep.Batch(proxy, precision=precision, graph=True) as b:
         for variant in range(sep.Batch.variant_count()):
             for bpc in blocks_per_cu:
@@ -76,10 +81,26 @@ def tune(groups, vectors: Sequence, precision: int, proxy_vectors: int = 1, reps
                 table.append((variant, bpc, best))
                 if log:
                     log(f"tune variant {variant} blocks/CU {bpc}: {best:.3f} ms")
-    v, bpc, ms = min(table, key=lambda t: t[2])
+            spills[variant] = b.variant_scratch(variant)
+    v, bpc, ms = pick(table, spills)
     choice = {"variant": v, "blocks_per_cu": bpc, "ms": ms,
-              "table": [list(t) for t in table], "tuned_at": time.time()}
+              "table": [list(t) for t in table], "scratch_bytes": spills[v],
+              "tuned_at": time.time()}
     if use_cache:
         cache[sig] = choice
         save_cache(cache, cache_path)
     return choice
+
+
+def pick(table: Sequence[Tuple[int, int, float]], spills: Dict[int, int],
+         margin: float = SPILL_MARGIN) -> Tuple[int, int, float]:
+    """The fastest (variant, blocks_per_cu, ms) of table among variants that
+    spill nothing (spills[v] == 0), unless the fastest overall is more than
+    ``margin`` faster than that."""
+    best = min(table, key=lambda t: t[2])
+    clean = [t for t in table if spills.get(t[0], 0) == 0]
+    if clean:
+        c = min(clean, key=lambda t: t[2])
+        if c[2] <= best[2] * (1.0 + margin):
+            return c
+    return best

@@ -56,6 +56,9 @@ public:
     bool query_done();               // non-blocking completion test
     double elapsed_ms();             // between record_start and record_end
     void set_variant(int variant, int blocks_per_cu) { variant_ = variant; blocks_per_cu_ = blocks_per_cu; }
+    // Largest per-work-item scratch (register spill) size of the integral
+    // kernels the planned batch would launch at this variant.
+    int variant_scratch(int variant, size_t* bytes) const;
     int n_slots() const { return (int)slot_n_regions_.size(); }
     size_t scratch_bytes() const;
     hipStream_t stream() const { return stream_; }

@@ -286,6 +286,21 @@ void BatchRunner::set_constants(const std::vector<Slot>& slots)
     for (size_t s = 0; s < slots.size(); s++) h_consts_.p[s] = slots[s].k;
 }
 
+int BatchRunner::variant_scratch(int variant, size_t* bytes) const
+{
+    size_t most = 0;
+    for (const Group& g : igroups_) {
+        size_t b = 0;
+        MW_HIP_CHECK(persistent_scratch_bytes(g.bg, mixed_ && g.bg == kBgHernquist,
+                                              g.bg == kBgHernquist ? variant : 0,
+                                              g.split ? kLaunchSplitOnly : g.exact ? kLaunchExactOnly : kLaunchAny,
+                                              &b));
+        most = std::max(most, b);
+    }
+    *bytes = most;
+    return MW_OK;
+}
+
 int BatchRunner::enqueue(bool kernel_events)
 {
     const int NI = integral_comps(), NS = star_comps();
@@ -468,6 +483,14 @@ int mw_batch_set_variant(mw_batch* b, int variant, int blocks_per_cu)
     b->runner.set_variant(variant, blocks_per_cu);
     b->drop_graph();
     return MW_OK;
+}
+
+int mw_batch_variant_scratch(const mw_batch* b, int variant, size_t* bytes)
+{
+    if (!b || !bytes || variant < 0 || variant >= persistent_variants()) return fail(MW_E_INVALID, "bad arguments");
+    if (!b->planned) return fail(MW_E_INVALID, "the batch has not run yet: its kernels are chosen by its first run");
+    MW_HIP_CHECK(hipSetDevice(b->device));
+    return b->runner.variant_scratch(variant, bytes);
 }
 
 int mw_batch_run(mw_batch* b, const double* params, mw_result* results)

@@ -36,6 +36,9 @@ hipError_t launch_persistent(int bg, bool mixed, int variant, int grid, const Re
                              int n_items, int total_blocks, const EvalConst* d_consts,
                              double* d_partials, unsigned* d_counter, hipStream_t stream,
                              int layouts = kLaunchAny);
+// Per-work-item scratch (register spill) bytes of the kernel that
+// launch_persistent runs for these arguments.
+hipError_t persistent_scratch_bytes(int bg, bool mixed, int variant, int layouts, size_t* bytes);
 hipError_t launch_reset_counters(unsigned* d_counters, int n, hipStream_t stream);
 hipError_t launch_star_any(int bg, bool mixed, const StarLaunch* d_items, int n_items, int n_blocks,
                            const EvalConst* d_consts, double* d_partials, hipStream_t stream);

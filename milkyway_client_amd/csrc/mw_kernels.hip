@@ -834,6 +834,12 @@ __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
         const int r_lo = valid ? (int)(lo_l < 0 ? 0 : (lo_l > R ? R : lo_l)) : 0;
         const int r_hi = valid ? (int)(hi_l < 0 ? 0 : (hi_l > R ? R : hi_l)) : 0;
 
+        // The column's sums over its r-steps: Kahan-compensated in FP64
+        // (the reference's order, evaluation_optimized.c:504-519); in mixed
+        // precision plain FP64 sums of positive terms, within (R - 1) 2^-53
+        // relative (8e-14 at 700 r-steps, far below the FP32 terms' own
+        // error), which frees the compensation's registers: the kernel then
+        // runs 4 waves per SIMD without spills.
         double acc[1 + S], cmp[1 + S];
 #pragma unroll
         for (int c = 0; c <= S; c++) acc[c] = cmp[c] = 0.0;
@@ -886,9 +892,15 @@ __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
                     bgv = bg * f;
                 }
                 if (k >= r_lo && k < r_hi) {
-                    kahan_add(acc[0], cmp[0], bgv);
+                    if constexpr (MIXED) {
+                        acc[0] += bgv;
 #pragma unroll
-                    for (int s = 0; s < S; s++) kahan_add(acc[1 + s], cmp[1 + s], st[s] * f);
+                        for (int s = 0; s < S; s++) acc[1 + s] = fma(st[s], f, acc[1 + s]);
+                    } else {
+                        kahan_add(acc[0], cmp[0], bgv);
+#pragma unroll
+                        for (int s = 0; s < S; s++) kahan_add(acc[1 + s], cmp[1 + s], st[s] * f);
+                    }
                 }
             }
         }
@@ -1583,17 +1595,30 @@ static PersistentFn pick_persistent_split(int v)
     return integral_persistent_split<kBgHernquist, 1>;
 }
 
+static PersistentFn pick_persistent_for(int bg, bool mixed, int variant, int layouts)
+{
+    return (layouts == kLaunchSplitOnly && mixed && bg == kBgHernquist)
+               ? pick_persistent_split(variant)
+               : pick_persistent(bg, mixed, variant, layouts != kLaunchExactOnly);
+}
+
 hipError_t launch_persistent(int bg, bool mixed, int variant, int grid, const RegionLaunch* d_items,
                              int n_items, int total_blocks, const EvalConst* d_consts,
                              double* d_partials, unsigned* d_counter, hipStream_t stream, int layouts)
 {
     if (total_blocks <= 0) return hipSuccess;
-    const PersistentFn fn = (layouts == kLaunchSplitOnly && mixed && bg == kBgHernquist)
-                                ? pick_persistent_split(variant)
-                                : pick_persistent(bg, mixed, variant, layouts != kLaunchExactOnly);
-    hipLaunchKernelGGL(fn, dim3(grid), dim3(kBlock), 0, stream, d_items, n_items, total_blocks, d_consts,
-                       d_partials, d_counter);
+    hipLaunchKernelGGL(pick_persistent_for(bg, mixed, variant, layouts), dim3(grid), dim3(kBlock), 0, stream,
+                       d_items, n_items, total_blocks, d_consts, d_partials, d_counter);
     return hipGetLastError();
+}
+
+hipError_t persistent_scratch_bytes(int bg, bool mixed, int variant, int layouts, size_t* bytes)
+{
+    hipFuncAttributes attr;
+    const hipError_t e = hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(
+                                                  pick_persistent_for(bg, mixed, variant, layouts)));
+    if (e == hipSuccess) *bytes = attr.localSizeBytes;
+    return e;
 }
 
 hipError_t launch_reset_counters(unsigned* d_counters, int n, hipStream_t stream)

@@ -39,6 +39,7 @@ EXPORTS = [
     "mw_ctx_region_integrals", "mw_ctx_last_timing", "mw_ctx_resident_bytes",
     "mw_ctx_destroy", "mw_last_error", "mw_device_count", "mw_device_synchronize",
     "mw_batch_create", "mw_batch_run", "mw_batch_set_variant", "mw_batch_variant_count",
+    "mw_batch_variant_scratch",
     "mw_batch_vector_count", "mw_batch_last_timing", "mw_batch_destroy",
     "mw_server_create", "mw_server_add_wedge", "mw_server_submit", "mw_server_poll",
     "mw_server_wait", "mw_server_get_stats", "mw_server_destroy",
@@ -157,6 +158,8 @@ def load_library() -> C.CDLL:
     L.mw_batch_set_variant.restype = C.c_int
     L.mw_batch_variant_count.argtypes = []
     L.mw_batch_variant_count.restype = C.c_int
+    L.mw_batch_variant_scratch.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_size_t)]
+    L.mw_batch_variant_scratch.restype = C.c_int
     L.mw_batch_vector_count.argtypes = [C.c_void_p]
     L.mw_batch_vector_count.restype = C.c_int
     L.mw_batch_last_timing.argtypes = [C.c_void_p, C.POINTER(mw_timing)]
@@ -436,6 +439,13 @@ class Batch:
 
     def set_variant(self, variant: int, blocks_per_cu: int):
         _check(load_library().mw_batch_set_variant(self.handle, variant, blocks_per_cu))
+
+    def variant_scratch(self, variant: int) -> int:
+        """Scratch (register spill) bytes per work-item of this batch's
+        integral kernels at the given variant."""
+        n = C.c_size_t()
+        _check(load_library().mw_batch_variant_scratch(self.handle, variant, C.byref(n)))
+        return int(n.value)
 
     def run(self, vectors) -> List[Result]:
         """vectors: one array per context, (n_vectors, n_params) each."""

@@ -94,3 +94,17 @@ def test_batch_rejects_bad_input(wedges):
     with sep.Batch([(ctx, 1)]) as b:
         with pytest.raises(sep.MwError):
             b.set_variant(99, 1)
+
+
+def test_variant_scratch_query(wedges):
+    """mw_batch_variant_scratch reports each variant's spill bytes once the
+    batch has planned its launch (its first run), and refuses before."""
+    groups = [(c, len(v)) for c, v, *_ in wedges]
+    with sep.Batch(groups, precision=sep.PREC_MIXED, graph=False) as b:
+        with pytest.raises(sep.MwError):
+            b.variant_scratch(0)
+        b.run([v for _, v, *_ in wedges])
+        sizes = [b.variant_scratch(v) for v in range(sep.Batch.variant_count())]
+        assert all(isinstance(s, int) and 0 <= s < 1 << 20 for s in sizes)
+        with pytest.raises(sep.MwError):
+            b.variant_scratch(sep.Batch.variant_count())
