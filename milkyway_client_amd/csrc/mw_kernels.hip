@@ -765,6 +765,14 @@ __device__ __forceinline__ void integral_block_split(const RegionLaunch& it, lon
 // choice does not change a result's bits.
 constexpr int kMaxGroup = 8;
 constexpr int kStreamGroup = kMaxGroup;
+// Rows of fewer convolution points add each term in FP64 instead
+// (point_terms).  The FP32 weights' rounding is the same in every column,
+// and the groups' additions drop tail terms downwards, so in an integral
+// over many sights these errors average only over the points of a row:
+// at 120 points the groups leave at most ~4e-9 of an integral, at 8-64
+// points up to ~1.4e-8, against ~2e-10 for terms added in FP64.  Such
+// coarse rows are cheap, and the kernels hold both loops.
+constexpr int kGroupMinConv = 100;
 
 template <int S, int BG, int G>
 __device__ __forceinline__ void stream_group(const double2* row, const float* row32, const Sight<S>& v,
@@ -868,12 +876,18 @@ __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
                 double bg = 0.0;
                 double st[kMaxStreams] = { 0.0, 0.0, 0.0, 0.0 };
                 const double2* row = tile + kk * n;
-                if constexpr (MIXED) {
+                if (MIXED && n >= kGroupMinConv) {
                     const float* row32 = tile32 + kk * n;
                     int i = 0;
                     for (; i + kStreamGroup <= n; i += kStreamGroup)
                         stream_group<S, BG, kStreamGroup>(row + i, row32 + i, sight, K, s2, bg, st);
                     if (i < n) stream_group<S, BG, 0>(row + i, row32 + i, sight, K, s2, bg, st, n - i);
+                } else if constexpr (MIXED) {
+#pragma unroll UNROLL
+                    for (int i = 0; i < n; i++) {
+                        const double2 p = row[i];
+                        point_terms<S, BG, true, true>(p.x, p.y, sight, K, bg, st, s2);
+                    }
                 } else {
 #pragma unroll UNROLL
                     for (int i = 0; i < n; i++) {

@@ -65,3 +65,19 @@ def test_mixed_sweep_parameter_vectors():
     print("worst over 32 vectors:", {k: f"{v:.2e}" for k, v in worst.items()})
     assert worst["lnP"] <= LNP_TOL
     assert worst["I_bg"] <= INT_REL_TOL and worst["I_st"] <= INT_REL_TOL
+
+
+@pytest.mark.parametrize("n_conv", [7, 9, 17, 64, 100, 101, 107, 120])
+def test_mixed_group_remainders(n_conv):
+    """Rows of at least 100 convolution points sum their stream terms in
+    FP32 groups of 8 points, the points left at a row's end in a shorter
+    group; coarser rows add each term in FP64.  Counts on both sides of the
+    switch and with 4, 5, 3 and 0 points left over, 4 streams, 4 seeded
+    vectors each, against the oracle."""
+    shape = synth.WedgeShape(40, 48, 24, n_conv, 1500)
+    p, stars = synth.workload(5, shape, 4)
+    errs = mixed_errors(p, stars, synth.parameter_vectors(7, p, 4))
+    worst = {k: max(e[k] for e in errs) for k in errs[0]}
+    print(f"n = {n_conv}:", {k: f"{v:.2e}" for k, v in worst.items()})
+    assert worst["lnP"] <= LNP_TOL
+    assert worst["I_bg"] <= INT_REL_TOL and worst["I_st"] <= INT_REL_TOL
