@@ -740,9 +740,6 @@ __device__ __forceinline__ void integral_block_split(const RegionLaunch& it, lon
 }
 
 // ---------------------------------------------------------------------------
-// Integral: one logical block = 256 consecutive columns of one region of one
-// evaluation slot; the block's double-double partials go to prow.
-// ---------------------------------------------------------------------------
 // Mixed precision on the exact tables: G consecutive convolution points of
 // one r-step (G = 0: the m < kMaxGroup points left at a row's end).  Each
 // point's background term goes to bg in FP64 (bg_point_mixed); each stream
@@ -752,15 +749,13 @@ __device__ __forceinline__ void integral_block_split(const RegionLaunch& it, lon
 // FP32 FMA, against a conversion to FP64 and an FP64 FMA, plus one
 // conversion and one FP64 add per group.
 // Error of a group's sum: each of the G products and G - 1 additions rounds
-// to nearest (relative 2^-24, unbiased).  An addition drops, always
-// downwards, a term under half an ulp of its running sum, that is under
-// 2^-24 of the group's largest term: a bias below (G - 1) 2^-24 of the
-// group, and only where one group's terms span 24 octaves.  Neighbouring
-// convolution points' terms differ by a factor 2^(2 |t| dt), dt the step of
-// t between them, so that happens only far out in the Gaussian's tail, whose
-// groups are negligible against the sight's peak.  qw32 carries a relative
-// error below 2^-24 per weight, the same in every column and parameter
-// vector and of random sign across the thousands of points a Gaussian spans.
+// to nearest (relative 2^-24).  An addition drops, always downwards, a term
+// under half an ulp of its running sum, that is under 2^-24 of the group's
+// largest term: a bias below (G - 1) 2^-24 of the group, and only where one
+// group's terms span 24 octaves.  Neighbouring convolution points' terms
+// differ by a factor 2^(2 |t| dt), dt the step of t between them, so that
+// happens only in a Gaussian's tail.  qw32 carries a relative error below
+// 2^-24 per weight, the same in every column and parameter vector.
 // The group size is the same in every kernel variant, so that the tuner's
 // choice does not change a result's bits.
 constexpr int kMaxGroup = 8;
@@ -801,6 +796,9 @@ __device__ __forceinline__ void stream_group(const double2* row, const float* ro
 }
 
 // ---------------------------------------------------------------------------
+// Integral: one logical block = 256 consecutive columns of one region of one
+// evaluation slot; the block's double-double partials go to prow.
+//
 // REC: the mixed kernels' recurrence and split layouts compiled in (items
 // with grid_step > 0); without them the kernel holds the exact tables' loop
 // alone, and its register allocation is that loop's.
@@ -868,7 +866,7 @@ __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
             for (int e = threadIdx.x; e < kt * n; e += kBlock) {
                 const double2 v = rq[(long)k0 * n + e];
                 tile[e] = v;
-                if constexpr (MIXED) tile32[e] = (float)v.y;
+                if (MIXED && n >= kGroupMinConv) tile32[e] = (float)v.y;
             }
             __syncthreads();
             for (int kk = 0; kk < kt; kk++) {
@@ -876,17 +874,19 @@ __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
                 double bg = 0.0;
                 double st[kMaxStreams] = { 0.0, 0.0, 0.0, 0.0 };
                 const double2* row = tile + kk * n;
-                if (MIXED && n >= kGroupMinConv) {
-                    const float* row32 = tile32 + kk * n;
-                    int i = 0;
-                    for (; i + kStreamGroup <= n; i += kStreamGroup)
-                        stream_group<S, BG, kStreamGroup>(row + i, row32 + i, sight, K, s2, bg, st);
-                    if (i < n) stream_group<S, BG, 0>(row + i, row32 + i, sight, K, s2, bg, st, n - i);
-                } else if constexpr (MIXED) {
+                if constexpr (MIXED) {
+                    if (n >= kGroupMinConv) {
+                        const float* row32 = tile32 + kk * n;
+                        int i = 0;
+                        for (; i + kStreamGroup <= n; i += kStreamGroup)
+                            stream_group<S, BG, kStreamGroup>(row + i, row32 + i, sight, K, s2, bg, st);
+                        if (i < n) stream_group<S, BG, 0>(row + i, row32 + i, sight, K, s2, bg, st, n - i);
+                    } else {
 #pragma unroll UNROLL
-                    for (int i = 0; i < n; i++) {
-                        const double2 p = row[i];
-                        point_terms<S, BG, true, true>(p.x, p.y, sight, K, bg, st, s2);
+                        for (int i = 0; i < n; i++) {
+                            const double2 p = row[i];
+                            point_terms<S, BG, true, true>(p.x, p.y, sight, K, bg, st, s2);
+                        }
                     }
                 } else {
 #pragma unroll UNROLL
