@@ -96,3 +96,26 @@ def test_bench_vector_sets_are_fresh():
     assert all(x.shape == (16, plist[0][0].optimized_parameter_count()) for x in a)
     assert not any(np.array_equal(x, y) for x, y in zip(a, b))
     assert all(np.array_equal(x, y) for x, y in zip(a, bench.vector_set(0, plist)))
+
+
+def test_bench_reads_the_headline_counters():
+    import bench
+    c = bench.measured_counters("integral_persistent_mixed_batch64")
+    assert c is not None and c["source"].startswith("profiles/")
+    for k in ("valu_busy", "valu_instructions_per_evaluation", "cycles_per_valu_instruction",
+              "hardware_tflops"):
+        assert c[k] > 0
+    assert bench.measured_traffic("integral_persistent_mixed_batch64") > 0
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """One GPU per run: --gpus 2 exits with status 2 and says why, before
+    touching a device."""
+    import subprocess
+    import sys
+    from pathlib import Path
+    repo = Path(__file__).resolve().parents[1]
+    r = subprocess.run([sys.executable, str(repo / "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
