@@ -25,8 +25,10 @@
 // the fractional part of the exponent) and keeps FP64 for the positions
 // (lbr -> xyz), the squared stream-axis distance, the background (its 1/sqrt
 // and 1/x from the FP64 estimates with one Newton step: as cheap as FP32
-// estimates with their conversions, and exact to FP64), every accumulation
-// and the per-star log.
+// estimates with their conversions, and exact to FP64), the per-star log and
+// every accumulation but one: in rows of at least kGroupMinConv convolution
+// points, the integral pass sums each stream's terms over groups of 8 points
+// in FP32 before adding them in FP64 (stream_group).
 #include <hip/hip_runtime.h>
 
 #include "mw_internal.h"
