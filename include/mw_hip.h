@@ -46,7 +46,8 @@ extern "C" {
 
 /* Precision modes */
 #define MW_PREC_FP64 0      /* FP64 throughout */
-#define MW_PREC_MIXED 1     /* FP32 exp for the streams; FP64 positions, distances, background, sums, logs */
+#define MW_PREC_MIXED 1     /* FP32 exp for the streams (their terms summed over 8 points in FP32 in rows
+                               of >= 100 points); FP64 positions, distances, background, other sums, logs */
 /* MW_PREC_MIXED on a resampled quadrature (context, batch and server calls;
  * not the drop-in entry points).  The reference's integral sums, per sight,
  * r_steps x convolve weighted densities at distances that are the same for
