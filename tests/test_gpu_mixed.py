@@ -83,11 +83,12 @@ def test_mixed_group_remainders(n_conv):
     assert worst["I_bg"] <= INT_REL_TOL and worst["I_st"] <= INT_REL_TOL
 
 
-@pytest.mark.parametrize("n_conv, streams", [(128, 1), (200, 2), (150, 3)])
+@pytest.mark.parametrize("n_conv, streams", [(128, 0), (128, 1), (200, 2), (150, 3)])
 def test_mixed_group_sums_other_stream_counts(n_conv, streams):
-    """The grouped rows with 1, 2 and 3 streams and convolutions beyond the
-    reference files' 120 points (200 fills the LDS tile with five rows),
-    4 seeded vectors each, against the oracle."""
+    """The grouped rows with no stream (the background alone) and with 1, 2
+    and 3 streams, at convolutions beyond the reference files' 120 points
+    (200 fills the LDS tile with five rows), 4 seeded vectors each, against
+    the oracle."""
     shape = synth.WedgeShape(40, 48, 24, n_conv, 1500)
     p, stars = synth.workload(11 + streams, shape, streams)
     errs = mixed_errors(p, stars, synth.parameter_vectors(13, p, 4))
