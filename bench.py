@@ -143,6 +143,19 @@ def first_rows(p, rows: int):
     return q
 
 
+def rows_share(p, rank: int, ranks: int):
+    """Copy of p whose regions keep rank's contiguous share of their mu rows
+    (rows [rank * m / ranks, (rank + 1) * m / ranks) of each region's m, same
+    cells): one GPU's part of a row-aligned split over `ranks` GPUs, as the
+    flat calculation-index split of parameters.c:502-522 gives it."""
+    q = copy.deepcopy(p)
+    for g in q.regions:
+        step = (g.mu_max - g.mu_min) / g.mu_steps
+        k0, k1 = rank * g.mu_steps // ranks, (rank + 1) * g.mu_steps // ranks
+        g.mu_min, g.mu_max, g.mu_steps = g.mu_min + k0 * step, g.mu_min + k1 * step, k1 - k0
+    return q
+
+
 def cpu_seconds_per_eval(p, stars, rows: int, n_star_sample: int, threads: int):
     """Seconds of one full evaluation of (p, stars) by the oracle (threads 0:
     the serial restatement of the reference; else its OpenMP mode over mu
@@ -356,6 +369,36 @@ def reference_size_fp64(steps: int = 20, warmup: int = 3) -> dict:
             "ms_per_eval_p99": round(1e3 * float(np.percentile(lat, 99)), 3),
             "workload": "70x160x64 (r x mu x nu), n=30, 1 stream, 10k stars", "warmup": warmup,
             "repetitions": steps}
+
+
+def one_gpu_share(full_ms: float, ranks: int = 8, steps: int = 10, warmup: int = 3) -> dict:
+    """One GPU's share of the full-size evaluation split over `ranks` GPUs
+    (rank 0: its mu rows of both regions and 1/ranks of the stars), at its
+    real shape, FP64 and mixed; the composing exchange of 1 + S doubles per
+    region is not part of it (one GPU here)."""
+    p, stars = synth.workload(1, synth.FULL_SIZE, synth.FULL_SIZE_STREAMS)
+    q = rows_share(p, 0, ranks)
+    sh = stars[: len(stars) // ranks]
+    vecs = synth.parameter_vectors(1, p, 4)
+    out = {"workload": (f"rank 0 of {ranks}: " + " + ".join(
+        f"{g.r_steps}x{g.mu_steps}x{g.nu_steps}" for g in q.regions)
+        + f" (r x mu x nu), n={q.convolve}, {synth.FULL_SIZE_STREAMS} streams, {len(sh)} stars")}
+    with sep.Context(ParamsHolder(q), StarsHolder(sh), timing=True) as ctx:
+        for name, prec in (("fp64", sep.PREC_FP64), ("mixed", sep.PREC_MIXED)):
+            for i in range(warmup):
+                ctx.eval(vecs[i % 4], precision=prec)
+            lat, kern = [], []
+            for i in range(steps):
+                s0 = time.perf_counter()
+                ctx.eval(vecs[i % 4], precision=prec)
+                lat.append(time.perf_counter() - s0)
+                kern.append(ctx.timing().integral_ms)
+            out[f"{name}_ms_per_eval_median"] = round(1e3 * float(np.median(lat)), 3)
+            out[f"{name}_integral_kernel_ms_median"] = round(float(np.median(kern)), 3)
+    out["fp64_efficiency_vs_full_over_ranks"] = round(full_ms / ranks / out["fp64_ms_per_eval_median"], 3)
+    out["what"] = ("the full-size FP64 evaluation's time / ranks over the share's: below 1 where the "
+                   "share's blocks no longer fill the GPU")
+    return out
 
 
 def full_size_mixed_32(reps: int = 3) -> dict:
@@ -610,6 +653,9 @@ def bench_batch(args):
         out["reference_size_fp64"] = reference_size_fp64()
         out["full_size_mixed_32"] = full_size_mixed_32()
         log(f"  {out['reference_size_fp64']['value']} evals/s; {out['full_size_mixed_32']['value']} evals/s")
+        log("one GPU's share of the full-size evaluation split over 8 GPUs")
+        out["one_gpu_share_of_8"] = one_gpu_share(full["ms_per_eval_median"])
+        log(f"  {out['one_gpu_share_of_8']}")
     if not args.no_serving:
         log("serving run")
         out["serving"] = serving_run(args.serving_requests, args.serving_rates, choice["variant"],

@@ -189,7 +189,8 @@ int mw_ctx_last_timing(const mw_ctx* ctx, mw_timing* out);
  * output.  Its working memory (constants, work items, block partials,
  * reduced rows) is a fixed array of MW_NOALLOC_SCRATCH_BYTES that every GPU
  * the library runs on holds until the process ends; an evaluation that needs
- * more (about 80 bytes per 256 sight columns and 112 per 256 stars) fails
+ * more (about 80 bytes per logical block -- a 256-column tile, or one of its
+ * r-windows on a region of few tiles -- and 112 per 256 stars) fails
  * with MW_E_INVALID.  Calls on one GPU share that array: calls that overlap
  * on one GPU race, so the caller serialises them.  stream: a hipStream_t, or
  * NULL for the null stream; the call waits for it before returning. */

@@ -90,6 +90,7 @@ int build_context(mw_ctx* c, const std::vector<RegionGeom>& geoms, int n_conv, i
         rd->n_conv = n_conv;
         rd->n_cols = t.n_cols;
         rd->blocks = (int)((t.n_cols + kBlock - 1) / kBlock);
+        rd->r_chunks = region_r_chunks(rd->blocks, g.r_steps);
         int rc;
         if ((rc = rd->rq.reserve(t.rq.size()))) return rc;
         if ((rc = rd->rstep.reserve(t.rstep.size()))) return rc;
@@ -225,8 +226,11 @@ int run_plan(mw_ctx* c, Plan& P)
             }
             L.slot = pr.slot;
             L.out_index = order[oi];
+            // r-windows on the reference's tables only; resampled grids run whole columns
+            L.r_chunks = level < 0 ? rd.r_chunks : 1;
+            const int blocks = rd.blocks * L.r_chunks;
             L.block_begin = g.n_blocks;          // block index within the group's launch
-            L.block_count = rd.blocks;
+            L.block_count = blocks;
             L.n_streams = nS;
             L.r_lo = rd.r_min;
             L.r_hi = rd.r_max;
@@ -243,12 +247,12 @@ int run_plan(mw_ctx* c, Plan& P)
             points_bg += (level >= 0 ? rd.n_cols * tr.bg_nodes : (pr.end - pr.begin) * (long)rd.n_conv);
             ReduceItem& R = c->h_rred.p[item];
             R.block_begin = total_iblocks;       // global partial index
-            R.block_count = rd.blocks;
+            R.block_count = blocks;
             R.out_index = order[oi];
             R.pad = 0;
             g.n_items++;
-            g.n_blocks += rd.blocks;
-            total_iblocks += rd.blocks;
+            g.n_blocks += blocks;
+            total_iblocks += blocks;
             item++;
         }
         if ((rc = c->ipart.reserve((size_t)total_iblocks * NI * 2))) return rc;

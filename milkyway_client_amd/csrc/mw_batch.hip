@@ -76,7 +76,8 @@ size_t BatchRunner::scratch_bytes_for(const std::vector<Slot>& slots)
     for (const Slot& s : slots) {
         b += sizeof(EvalConst) + sizeof(StarLaunch) + 2 * sizeof(ReduceItem) + 2 * sizeof(int);
         for (const auto& r : s.ctx->regions)
-            b += sizeof(RegionLaunch) + sizeof(ReduceItem) + (size_t)r->blocks * NI * 2 * 8 + NI * 8;
+            b += sizeof(RegionLaunch) + sizeof(ReduceItem) + (size_t)r->blocks * r->r_chunks * NI * 2 * 8 +
+                 NI * 8;
         b += (size_t)((s.ctx->n_stars + kBlock - 1) / kBlock) * NS * 2 * 8 + NS * 8;
     }
     return b;
@@ -141,7 +142,9 @@ int BatchRunner::plan(const std::vector<Slot>& slots, int precision)
             L.calc_end = rd->n_cols * (long)tr.r_steps;
             L.slot = s;
             L.out_index = row++;
-            L.block_count = rd->blocks;
+            // r-windows on the reference's tables only; resampled grids run whole columns
+            L.r_chunks = level < 0 ? rd->r_chunks : 1;
+            L.block_count = rd->blocks * L.r_chunks;
             L.n_streams = slots[s].k.n_streams;
             L.r_lo = rd->r_min;
             L.r_hi = rd->r_max;
@@ -160,7 +163,7 @@ int BatchRunner::plan(const std::vector<Slot>& slots, int precision)
             // cost per block: stream terms and background evaluations (about
             // 6.5 stream terms each), or per point on the other layouts
             const double cost = layout == kLayoutSplit ? terms + 6.5 * tr.bg_count
-                                                       : (double)L.r_steps * L.n_conv * (2.0 + L.n_streams);
+                                                       : (double)L.r_steps / L.r_chunks * L.n_conv * (2.0 + L.n_streams);
             items.push_back(It{ L, cost });
         }
         levels_.push_back(star_level(*c, slots[s].k));

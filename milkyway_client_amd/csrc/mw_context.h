@@ -279,11 +279,30 @@ struct ResampledDev {
     DevBuf<double> w;   // (r, weight) tables: the weights alone, contiguous (layout 4's streams)
 };
 
+// Windows of the r-steps per 256-column tile for a region's exact tables:
+// a region of few tiles would leave most of the GPU idle in a launch of its
+// own (a single evaluation of a small wedge, one GPU's share of a sharded
+// one), so its tiles are split along r until the region has about
+// kChunkTargetBlocks logical blocks, each window keeping at least
+// kChunkMinRSteps r-steps.  It depends on the region alone, so that every
+// launch of the region (single, batched, without allocation) sums the same
+// partials in the same order.
+constexpr int kChunkTargetBlocks = 2048;
+constexpr int kChunkMinRSteps = 4;
+inline int region_r_chunks(int tiles, int r_steps)
+{
+    if (tiles <= 0 || r_steps <= 0) return 1;
+    const int want = (kChunkTargetBlocks + tiles - 1) / tiles;
+    const int most = r_steps / kChunkMinRSteps;
+    return std::max(1, std::min(want, most));
+}
+
 struct RegionDev {
     RegionGeom geom;
     int n_conv = 0;
     long n_cols = 0;
-    int blocks = 0;
+    int blocks = 0;          // 256-column tiles
+    int r_chunks = 1;        // windows of the r-steps per tile on the exact tables (region_r_chunks)
     DevBuf<double> rq, col;
     DevBuf<RStep> rstep;
     // Host copies of the exact r-step tables, and what the choice of a

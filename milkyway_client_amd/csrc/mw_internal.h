@@ -88,6 +88,8 @@ struct RegionLaunch {
     int slot;                  // evaluation-constant slot
     int out_index;             // index into the per-region result rows
     int block_begin, block_count;
+    int r_chunks;              // exact tables: windows of the r-steps per 256-column tile (block lb
+                               // takes tile lb / r_chunks, window lb % r_chunks); 0 or 1: one window
     int n_streams;             // streams of the slot's model (batched kernels)
     double r_lo, r_hi;         // exact tables: the smallest and largest r of the region's points
                                // (the mixed kernels shift each stream's exponent by its smallest

@@ -822,7 +822,13 @@ __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
     const EvalConst& K = consts[it.slot];
     const int R = it.r_steps, n = it.n_conv;
     const long C = it.n_cols;
-    const long col0 = lb * kBlock;
+    // Logical block lb: 256-column tile lb / chunks, its r-steps
+    // [k_begin, k_end) (window lb % chunks of chunks).
+    const int chunks = it.r_chunks > 1 ? it.r_chunks : 1;
+    const long tile_ix = lb / chunks;
+    const int window = (int)(lb - tile_ix * chunks);
+    const int k_begin = (int)((long)window * R / chunks), k_end = (int)((long)(window + 1) * R / chunks);
+    const long col0 = tile_ix * kBlock;
     const long col = col0 + threadIdx.x;
     const bool valid = col < C;
     const long cc = valid ? col : C - 1;
@@ -862,8 +868,8 @@ __device__ __forceinline__ void integral_block(const RegionLaunch& it, long lb,
 
         const int rt = kTileD2 / n;   // r-steps per LDS tile (host guarantees n <= kTileD2)
         const double2* rq = reinterpret_cast<const double2*>(it.rq);
-        for (int k0 = 0; k0 < R; k0 += rt) {
-            const int kt = (R - k0 < rt) ? R - k0 : rt;
+        for (int k0 = k_begin; k0 < k_end; k0 += rt) {
+            const int kt = (k_end - k0 < rt) ? k_end - k0 : rt;
             __syncthreads();
             for (int e = threadIdx.x; e < kt * n; e += kBlock) {
                 const double2 v = rq[(long)k0 * n + e];

@@ -124,12 +124,13 @@ extern "C" int mw_ctx_eval_noalloc(mw_ctx* c, const double* params, int precisio
         L.slot = 0;
         L.out_index = g;
         L.block_begin = iblocks;
-        L.block_count = rd.blocks;
+        L.r_chunks = rd.r_chunks;
+        L.block_count = rd.blocks * rd.r_chunks;
         L.n_streams = S;
         L.r_lo = rd.r_min;
         L.r_hi = rd.r_max;
-        ired[g] = ReduceItem{ iblocks, rd.blocks, g, 0 };
-        iblocks += rd.blocks;
+        ired[g] = ReduceItem{ iblocks, L.block_count, g, 0 };
+        iblocks += L.block_count;
     }
     const long N = c->n_stars;
     const int sblocks = N > 0 ? (int)((N + kBlock - 1) / kBlock) : 0;

@@ -119,3 +119,20 @@ def test_bench_refuses_more_than_one_gpu():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_bench_rows_share_partitions_every_region():
+    """The 8 ranks' shares tile each region's mu rows with the same cells."""
+    import bench
+    p, _ = synth.workload(1, synth.FULL_SIZE, 3)
+    shares = [bench.rows_share(p, r, 8) for r in range(8)]
+    for gi, g in enumerate(p.regions):
+        step = (g.mu_max - g.mu_min) / g.mu_steps
+        parts = [s.regions[gi] for s in shares]
+        assert sum(x.mu_steps for x in parts) == g.mu_steps
+        assert parts[0].mu_min == g.mu_min and parts[-1].mu_max == pytest.approx(g.mu_max)
+        for a, b in zip(parts, parts[1:]):
+            assert a.mu_max == pytest.approx(b.mu_min)
+        for x in parts:
+            assert (x.mu_max - x.mu_min) / x.mu_steps == pytest.approx(step)
+            assert (x.r_steps, x.nu_steps) == (g.r_steps, g.nu_steps)

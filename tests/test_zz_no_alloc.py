@@ -58,7 +58,7 @@ def test_noalloc_equals_ctx_eval_on_two_calls(precision):
 def test_noalloc_refuses_what_it_cannot_hold():
     sep.load_library()
     sep.require_device()
-    # the full-size wedge needs about 0.44 MB of scratch and fits; a model of
+    # the full-size wedge needs about 0.59 MB of scratch and fits; a model of
     # more than 4 streams and the resampled quadrature are refused
     p, stars = synth.workload(303, synth.WedgeShape(10, 12, 8, 10, 50), 5)
     with sep.Context(ParamsHolder(p), StarsHolder(stars)) as ctx:
