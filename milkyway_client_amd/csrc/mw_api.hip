@@ -170,7 +170,7 @@ int run_plan(mw_ctx* c, Plan& P)
     MW_HIP_CHECK(hipSetDevice(c->device));
     if ((rc = c->ensure_stream())) return rc;
     hipStream_t st = c->stream;
-    if ((rc = c->consts.reserve(n_slots)) || (rc = c->h_consts.reserve(n_slots))) return rc;
+    if ((rc = c->h_consts.reserve(n_slots))) return rc;
     std::memcpy(c->h_consts.p, P.K.data(), sizeof(EvalConst) * n_slots);
 
     // ---- integral work items, grouped by kernel variant ----------------------
@@ -182,9 +182,8 @@ int run_plan(mw_ctx* c, Plan& P)
     int total_iblocks = 0;
     long points = 0, points_rec = 0, points_bg = 0, stream_terms = 0;
     if (P.integrals && n_pr > 0) {
-        if ((rc = c->ritems.reserve(n_pr)) || (rc = c->h_ritems.reserve(n_pr)) ||
-            (rc = c->rred.reserve(n_pr)) || (rc = c->h_rred.reserve(n_pr)) ||
-            (rc = c->irows.reserve((size_t)n_pr * NI)) || (rc = c->h_irows.reserve((size_t)n_pr * NI)))
+        if ((rc = c->h_ritems.reserve(n_pr)) || (rc = c->h_rred.reserve(n_pr)) ||
+            (rc = c->h_irows.reserve((size_t)n_pr * NI)))
             return rc;
         std::vector<int> order(n_pr);
         for (int i = 0; i < n_pr; i++) order[i] = i;
@@ -263,9 +262,7 @@ int run_plan(mw_ctx* c, Plan& P)
     int total_sblocks = 0;
     const long n_star_pass = c->n_stars - P.star_begin;
     if (P.stars) {
-        if ((rc = c->sitems.reserve(n_slots)) || (rc = c->h_sitems.reserve(n_slots)) ||
-            (rc = c->sred.reserve(n_slots)) || (rc = c->h_sred.reserve(n_slots)) ||
-            (rc = c->srows.reserve((size_t)n_slots * NS)) ||
+        if ((rc = c->h_sitems.reserve(n_slots)) || (rc = c->h_sred.reserve(n_slots)) ||
             (rc = c->h_srows.reserve((size_t)n_slots * NS)))
             return rc;
         const int blocks = n_star_pass > 0 ? (int)((n_star_pass + kBlock - 1) / kBlock) : 0;
@@ -320,8 +317,7 @@ int run_plan(mw_ctx* c, Plan& P)
 
     // ---- device-side combination of regions per slot -------------------------
     if (P.device_finalize && P.stars && P.integrals) {
-        if ((rc = c->first.reserve(n_slots)) || (rc = c->count.reserve(n_slots)) ||
-            (rc = c->h_first.reserve(n_slots)) || (rc = c->h_count.reserve(n_slots)))
+        if ((rc = c->h_first.reserve(n_slots)) || (rc = c->h_count.reserve(n_slots)))
             return rc;
         for (int s = 0; s < n_slots; s++) {
             c->h_first.p[s] = -1;
@@ -334,56 +330,101 @@ int run_plan(mw_ctx* c, Plan& P)
         }
         for (int s = 0; s < n_slots; s++)
             if (c->h_first.p[s] < 0) return fail(MW_E_INVALID, "slot without regions");
-        MW_HIP_CHECK(hipMemcpyAsync(c->first.p, c->h_first.p, sizeof(int) * n_slots, hipMemcpyHostToDevice, st));
-        MW_HIP_CHECK(hipMemcpyAsync(c->count.p, c->h_count.p, sizeof(int) * n_slots, hipMemcpyHostToDevice, st));
     }
 
-    MW_HIP_CHECK(hipMemcpyAsync(c->consts.p, c->h_consts.p, sizeof(EvalConst) * n_slots, hipMemcpyHostToDevice, st));
-    if (P.integrals && n_pr > 0) {
-        MW_HIP_CHECK(hipMemcpyAsync(c->ritems.p, c->h_ritems.p, sizeof(RegionLaunch) * n_pr, hipMemcpyHostToDevice, st));
-        MW_HIP_CHECK(hipMemcpyAsync(c->rred.p, c->h_rred.p, sizeof(ReduceItem) * n_pr, hipMemcpyHostToDevice, st));
+    // ---- one upload, one download ----------------------------------------
+    // The constants and work items go up in one copy from one pinned block,
+    // and the region and star rows come back in one: each hipMemcpyAsync
+    // costs several microseconds of host time, as much as a small wedge's
+    // kernels.  Sections are 256-byte aligned at the same offsets on both
+    // sides.  The previous evaluation on this context synchronised its
+    // stream before returning, so the pinned block is free to refill.
+    const bool items = P.integrals && n_pr > 0;
+    const bool fin = P.device_finalize && P.stars && P.integrals;
+    size_t up = 0;
+    auto place = [&](size_t bytes) {
+        up = (up + 255) & ~size_t(255);
+        const size_t at = up;
+        up += bytes;
+        return at;
+    };
+    const size_t o_consts = place(sizeof(EvalConst) * n_slots);
+    const size_t o_ritems = items ? place(sizeof(RegionLaunch) * n_pr) : 0;
+    const size_t o_rred = items ? place(sizeof(ReduceItem) * n_pr) : 0;
+    const size_t o_sitems = P.stars ? place(sizeof(StarLaunch) * n_slots) : 0;
+    const size_t o_sred = P.stars ? place(sizeof(ReduceItem) * n_slots) : 0;
+    const size_t o_first = fin ? place(sizeof(int) * n_slots) : 0;
+    const size_t o_count = fin ? place(sizeof(int) * n_slots) : 0;
+    if ((rc = c->h_up.reserve(up)) || (rc = c->d_up.reserve(up))) return rc;
+    auto stage = [&](size_t at, const void* src, size_t bytes) { std::memcpy(c->h_up.p + at, src, bytes); };
+    stage(o_consts, c->h_consts.p, sizeof(EvalConst) * n_slots);
+    if (items) {
+        stage(o_ritems, c->h_ritems.p, sizeof(RegionLaunch) * n_pr);
+        stage(o_rred, c->h_rred.p, sizeof(ReduceItem) * n_pr);
     }
     if (P.stars) {
-        MW_HIP_CHECK(hipMemcpyAsync(c->sitems.p, c->h_sitems.p, sizeof(StarLaunch) * n_slots, hipMemcpyHostToDevice, st));
-        MW_HIP_CHECK(hipMemcpyAsync(c->sred.p, c->h_sred.p, sizeof(ReduceItem) * n_slots, hipMemcpyHostToDevice, st));
+        stage(o_sitems, c->h_sitems.p, sizeof(StarLaunch) * n_slots);
+        stage(o_sred, c->h_sred.p, sizeof(ReduceItem) * n_slots);
     }
+    if (fin) {
+        stage(o_first, c->h_first.p, sizeof(int) * n_slots);
+        stage(o_count, c->h_count.p, sizeof(int) * n_slots);
+    }
+    MW_HIP_CHECK(hipMemcpyAsync(c->d_up.p, c->h_up.p, up, hipMemcpyHostToDevice, st));
+    EvalConst* const d_consts = reinterpret_cast<EvalConst*>(c->d_up.p + o_consts);
+    const RegionLaunch* const d_ritems = reinterpret_cast<const RegionLaunch*>(c->d_up.p + o_ritems);
+    const ReduceItem* const d_rred = reinterpret_cast<const ReduceItem*>(c->d_up.p + o_rred);
+    const StarLaunch* const d_sitems = reinterpret_cast<const StarLaunch*>(c->d_up.p + o_sitems);
+    const ReduceItem* const d_sred = reinterpret_cast<const ReduceItem*>(c->d_up.p + o_sred);
+    const int* const d_first = reinterpret_cast<const int*>(c->d_up.p + o_first);
+    const int* const d_count = reinterpret_cast<const int*>(c->d_up.p + o_count);
+    // rows: the regions' [n_pr][NI], then the stars' [n_slots][NS]
+    const size_t n_irows = items ? (size_t)n_pr * NI : 0;
+    const size_t n_srows = P.stars ? (size_t)n_slots * NS : 0;
+    if ((rc = c->d_rows.reserve(std::max<size_t>(n_irows + n_srows, 1))) ||
+        (rc = c->h_rows.reserve(std::max<size_t>(n_irows + n_srows, 1))))
+        return rc;
+    double* const d_irows = c->d_rows.p;
+    double* const d_srows = c->d_rows.p + n_irows;
 
     if (timed) MW_HIP_CHECK(hipEventRecord(c->ev[0], st));
     for (const Group& g : igroups) {
-        MW_HIP_CHECK(launch_integral(g.S, g.bg, P.mixed, c->ritems.p + g.first_item, g.n_items,
-                                     g.n_blocks, c->consts.p, c->ipart.p + (size_t)g.first_block * NI * 2, st,
+        MW_HIP_CHECK(launch_integral(g.S, g.bg, P.mixed, d_ritems + g.first_item, g.n_items,
+                                     g.n_blocks, d_consts, c->ipart.p + (size_t)g.first_block * NI * 2, st,
                                      P.resample));
     }
     if (timed) MW_HIP_CHECK(hipEventRecord(c->ev[1], st));
     if (P.integrals && n_pr > 0) {
-        MW_HIP_CHECK(launch_reduce(false, c->rred.p, n_pr, c->ipart.p, c->irows.p, st));
-        if (P.device_finalize && P.stars)
-            MW_HIP_CHECK(launch_finalize(c->first.p, c->count.p, c->irows.p, c->consts.p, n_slots, st));
+        MW_HIP_CHECK(launch_reduce(false, d_rred, n_pr, c->ipart.p, d_irows, st));
+        if (fin) MW_HIP_CHECK(launch_finalize(d_first, d_count, d_irows, d_consts, n_slots, st));
     }
     if (timed) MW_HIP_CHECK(hipEventRecord(c->ev[2], st));
     for (const Group& g : sgroups) {
         if (g.n_blocks == 0) continue;
-        MW_HIP_CHECK(launch_star(g.S, g.bg, P.mixed, c->sitems.p + g.first_item, g.n_items, g.n_blocks,
-                                 c->consts.p, c->ratio.p, c->kw.p,
+        MW_HIP_CHECK(launch_star(g.S, g.bg, P.mixed, d_sitems + g.first_item, g.n_items, g.n_blocks,
+                                 d_consts, c->ratio.p, c->kw.p,
                                  c->spart.p + (size_t)g.first_block * NS * 2,
                                  P.log10p_host ? c->log10p.p : nullptr, c->n_stars, st));
     }
     if (timed) MW_HIP_CHECK(hipEventRecord(c->ev[3], st));
     if (P.stars) {
         if (total_sblocks > 0)
-            MW_HIP_CHECK(launch_reduce(true, c->sred.p, n_slots, c->spart.p, c->srows.p, st));
+            MW_HIP_CHECK(launch_reduce(true, d_sred, n_slots, c->spart.p, d_srows, st));
     }
     if (timed) MW_HIP_CHECK(hipEventRecord(c->ev[4], st));
-    if (P.integrals && n_pr > 0)
-        MW_HIP_CHECK(hipMemcpyAsync(c->h_irows.p, c->irows.p, sizeof(double) * n_pr * NI, hipMemcpyDeviceToHost, st));
-    if (P.stars && total_sblocks > 0)
-        MW_HIP_CHECK(hipMemcpyAsync(c->h_srows.p, c->srows.p, sizeof(double) * n_slots * NS, hipMemcpyDeviceToHost, st));
-    else if (P.stars)
-        std::memset(c->h_srows.p, 0, sizeof(double) * n_slots * NS);
+    // the star rows only where a star pass wrote them
+    const size_t n_down = n_irows + (total_sblocks > 0 ? n_srows : 0);
+    if (n_down > 0)
+        MW_HIP_CHECK(hipMemcpyAsync(c->h_rows.p, d_irows, sizeof(double) * n_down, hipMemcpyDeviceToHost, st));
     if (P.stars && P.log10p_host && c->n_stars > 0)
         MW_HIP_CHECK(hipMemcpyAsync(P.log10p_host, c->log10p.p, sizeof(double) * (n_slots / P.groups) * c->n_stars,
                                     hipMemcpyDeviceToHost, st));
     MW_HIP_CHECK(hipStreamSynchronize(st));
+    if (items) std::memcpy(c->h_irows.p, c->h_rows.p, sizeof(double) * n_irows);
+    if (P.stars) {
+        if (total_sblocks > 0) std::memcpy(c->h_srows.p, c->h_rows.p + n_irows, sizeof(double) * n_srows);
+        else std::memset(c->h_srows.p, 0, sizeof(double) * n_srows);
+    }
     if (timed) {
         float a = 0, b = 0, t = 0;
         MW_HIP_CHECK(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));

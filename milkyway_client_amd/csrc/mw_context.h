@@ -455,14 +455,15 @@ struct mw_ctx {
     std::unique_ptr<mw::ResampledDev> star_bg[mw::kBgLevels];
     mutable std::mutex star_bg_mu;
 
-    // per-evaluation buffers
-    mw::DevBuf<mw::EvalConst> consts;
-    mw::DevBuf<mw::RegionLaunch> ritems;
-    mw::DevBuf<mw::StarLaunch> sitems;
-    mw::DevBuf<mw::ReduceItem> rred, sred;
-    mw::DevBuf<double> ipart, spart, irows, srows, log10p;
+    // per-evaluation buffers: constants and work items staged on the host
+    // (h_*) and uploaded together in one block (h_up -> d_up); region and
+    // star rows downloaded together (d_rows -> h_rows, then h_irows, h_srows)
+    mw::DevBuf<unsigned char> d_up;
+    mw::HostBuf<unsigned char> h_up;
+    mw::DevBuf<double> d_rows;
+    mw::HostBuf<double> h_rows;
+    mw::DevBuf<double> ipart, spart, log10p;
     mw::DevBuf<double> chain;           // per-star mixture sums between stream groups
-    mw::DevBuf<int> first, count;
     mw::DevBuf<int> sep_assign;         // stream separation outputs
     mw::DevBuf<double> sep_xyz, sep_prob;
     mw::HostBuf<mw::EvalConst> h_consts;
