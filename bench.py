@@ -421,44 +421,61 @@ def full_size_mixed_32(reps: int = 3) -> dict:
             "accuracy": "tests/test_gpu_configs.py::test_full_size_mixed_32_vectors (per star vs FP64)"}
 
 
-def serving_run(n_requests: int, rates, variant: int, bpc: int):
-    """Poisson arrival streams at each rate over 8 resident wedges; requests
-    are resampled mixed precision with a 10% FP64 share.  Every rate serves
-    the same request sequence (its first max(n_requests, min(400, 2 rate))
-    requests), so that only the arrival rate changes between rows.  The
-    plain one-at-a-time FP64 server is run at the lowest rate only (it
-    saturates at about 10 requests/s)."""
+def serving_sweep(srv, ids, all_reqs, rates, n_requests, fast):
+    """Serves the first requests of all_reqs at each Poisson rate on the
+    running server; one row per rate, with the fast class (named by fast) and
+    the FP64 class reported apart."""
+    rows = []
+    for k, rate in enumerate(rates):
+        n = max(n_requests, min(SERVING_SET, int(2 * rate)))
+        reqs = all_reqs[:n]
+        arr = serving.poisson_arrivals(len(reqs), rate, seed=6000 + k)
+        b0 = srv.stats().batches
+        v = serving.run(srv, ids, reqs, arr)
+        st = srv.stats()
+        row = {"poisson_rate_per_s": rate, "requests": n,
+               "throughput_req_per_s": round(v["throughput"], 2),
+               "p50_ms": round(v["p50_ms"], 2), "p99_ms": round(v["p99_ms"], 2),
+               "batches": int(st.batches - b0)}
+        for prec, name in (fast, (sep.PREC_FP64, "fp64")):
+            if f"p50_ms_prec{prec}" in v:
+                row[f"{name}_p50_ms"] = round(v[f"p50_ms_prec{prec}"], 2)
+                row[f"{name}_p99_ms"] = round(v[f"p99_ms_prec{prec}"], 2)
+                row[f"{name}_requests"] = v[f"n_prec{prec}"]
+                row[f"{name}_mean_queue_ms"] = round(v[f"queue_ms_prec{prec}"], 2)
+                row[f"{name}_mean_service_ms"] = round(v[f"service_ms_prec{prec}"], 2)
+        log(f"  rate {rate}/s: {row}")
+        rows.append(row)
+    return rows
+
+
+def serving_run(n_requests: int, rates, exact_rates, variant: int, bpc: int):
+    """Poisson arrival streams at each rate over 8 resident wedges, 10% of
+    the requests FP64.  Two sweeps over one seeded request sequence (the same
+    wedges, vectors and FP64 flags): the other requests in resampled mixed
+    precision (`sweep`), then in mixed precision on the exact quadrature
+    (`exact_quadrature_sweep`, the reference's r-step x convolution points of
+    every cell).  Every rate serves the sequence's first
+    max(n_requests, min(400, 2 rate)) requests, so that only the arrival
+    rate changes between rows.  The plain one-at-a-time FP64 server is run
+    at the lowest rate only (it saturates at about 10 requests/s)."""
     wedges = serving.make_wedges(8, seed=4000)
     holders = [(ParamsHolder(p), StarsHolder(s)) for p, s, _ in wedges]
     out = {"wedges": "8 resident wedges from the batch shapes, 1-4 streams, 10% FP64 requests",
-           "requests": "one seeded sequence of 400 requests; each rate serves its first n",
-           "sweep": []}
-    all_reqs = serving.make_requests(wedges, SERVING_SET, seed=5000, fast_precision=sep.PREC_RESAMPLED)
+           "requests": "one seeded sequence of 400 requests; each rate serves its first n"}
+    resampled = serving.make_requests(wedges, SERVING_SET, seed=5000, fast_precision=sep.PREC_RESAMPLED)
+    exact = serving.make_requests(wedges, SERVING_SET, seed=5000, fast_precision=sep.PREC_MIXED)
     with sep.Server(max_inflight=3, max_batch=32, variant=variant, blocks_per_cu=bpc) as srv:
         ids = [srv.add_wedge(ph, sh) for ph, sh in holders]
-        for k, rate in enumerate(rates):
-            n = max(n_requests, min(SERVING_SET, int(2 * rate)))
-            reqs = all_reqs[:n]
-            arr = serving.poisson_arrivals(len(reqs), rate, seed=6000 + k)
-            b0 = srv.stats().batches
-            v = serving.run(srv, ids, reqs, arr)
-            st = srv.stats()
-            row = {"poisson_rate_per_s": rate, "requests": n,
-                   "throughput_req_per_s": round(v["throughput"], 2),
-                   "p50_ms": round(v["p50_ms"], 2), "p99_ms": round(v["p99_ms"], 2),
-                   "batches": int(st.batches - b0)}
-            for prec, name in ((sep.PREC_RESAMPLED, "resampled"), (sep.PREC_FP64, "fp64")):
-                if f"p50_ms_prec{prec}" in v:
-                    row[f"{name}_p50_ms"] = round(v[f"p50_ms_prec{prec}"], 2)
-                    row[f"{name}_p99_ms"] = round(v[f"p99_ms_prec{prec}"], 2)
-                    row[f"{name}_requests"] = v[f"n_prec{prec}"]
-                    row[f"{name}_mean_queue_ms"] = round(v[f"queue_ms_prec{prec}"], 2)
-                    row[f"{name}_mean_service_ms"] = round(v[f"service_ms_prec{prec}"], 2)
-            log(f"  rate {rate}/s: {row}")
-            out["sweep"].append(row)
+        out["sweep"] = serving_sweep(srv, ids, resampled, rates, n_requests,
+                                     (sep.PREC_RESAMPLED, "resampled"))
+        if exact_rates:
+            log("  exact quadrature:")
+            out["exact_quadrature_sweep"] = serving_sweep(srv, ids, exact, exact_rates, n_requests,
+                                                          (sep.PREC_MIXED, "mixed"))
         out["largest_batch"] = int(srv.stats().largest_batch)
     rate = rates[0]
-    reqs = all_reqs[:n_requests]
+    reqs = resampled[:n_requests]
     arr = serving.poisson_arrivals(len(reqs), rate, seed=6000)
     with sep.Server(max_inflight=1, max_batch=1) as srv:
         ids = [srv.add_wedge(ph, sh) for ph, sh in holders]
@@ -658,8 +675,8 @@ def bench_batch(args):
         log(f"  {out['one_gpu_share_of_8']}")
     if not args.no_serving:
         log("serving run")
-        out["serving"] = serving_run(args.serving_requests, args.serving_rates, choice["variant"],
-                                     choice["blocks_per_cu"])
+        out["serving"] = serving_run(args.serving_requests, args.serving_rates,
+                                     args.serving_exact_rates, choice["variant"], choice["blocks_per_cu"])
     if not args.no_cpu:
         log("CPU baseline (oracle, one core and all cores)")
         from oracle import oracle as O
@@ -743,6 +760,9 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=40)
     ap.add_argument("--serving-rates", type=float, nargs="+", default=[10.0, 50.0, 200.0, 800.0],
                     help="Poisson arrival rates (requests/s) of the serving sweep")
+    ap.add_argument("--serving-exact-rates", type=float, nargs="*", default=[5.0, 20.0, 80.0],
+                    help="Poisson arrival rates of the serving sweep on the exact quadrature "
+                         "(none: skip it)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
