@@ -2,7 +2,8 @@
 precision (evaluation_optimized.c:190-299, :424-753).
 
 Each case changes one thing the synthetic workloads never reach: a wedge of a
-single sight, a convolution of one point, a stream whose axis runs along the
+single sight, a wedge of one r-step, a convolution of one point and one of
+1024 (the most the library takes: one r-step per LDS tile), a stream whose axis runs along the
 wedge's central sight (the sights near it are almost parallel to the axis),
 a narrow stream centred 0.7 kpc from the Sun, and a model at the far edges
 of the reference's parameter bounds together with a second stream that
@@ -69,6 +70,8 @@ CASES = {
     "axis_along_the_central_sight": (LONG_R, 1, axis_along_central_sight),
     "narrow_stream_at_the_sun": (LONG_R, 2, narrow_at_the_sun),
     "bound_edges_and_a_far_stream": (LONG_R, 2, bound_edges),
+    "single_r_step": (synth.WedgeShape(1, 30, 12, 30, 300), 2, None),
+    "convolution_at_its_maximum": (synth.WedgeShape(6, 6, 4, 1024, 300), 2, None),
 }
 
 
@@ -91,3 +94,13 @@ def test_corner_against_the_oracle(case):
             print(case, prec, {k: f"{v:.2e}" for k, v in e.items()})
             assert e["I_bg"] <= int_tol and e["I_st"] <= int_tol, (case, prec, e)
             assert e["lnP"] <= lnp_tol and e["lnL"] <= lnp_tol, (case, prec, e)
+
+
+def test_convolution_beyond_its_maximum_is_refused():
+    """1025 convolution points exceed the LDS r-tile: the context refuses
+    the model with MW_E_INVALID and says why."""
+    p, stars = synth.workload(401, synth.WedgeShape(4, 4, 4, 1025, 50), 1)
+    with pytest.raises(sep.MwError) as e:
+        sep.Context(ParamsHolder(p), StarsHolder(stars))
+    assert e.value.code == sep.MW_E_INVALID
+    assert "convolve" in str(e.value)
