@@ -5,6 +5,8 @@ on the evaluator's result (mean log10 P), and every star's log10 P within
 1e-10 absolute.  Inputs: the reference's own test fixtures (parameter files,
 star subsets, CTest parameter vectors) and seeded synthetic wedges.
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -234,3 +236,27 @@ def test_full_size_geometry_slice():
     sep.calculate_likelihood(ph, es, sh)
     O.likelihood(ph2, sh2, es2)
     assert abs(es.es.prob_sum - es2.es.prob_sum) / len(stars) <= LNL_TOL
+
+
+def test_zero_stars():
+    """A star list of no stars.  The drop-in's integrals are the oracle's;
+    the star pass runs no star and leaves prob_sum 0, num_zero 0 and
+    current_star_point 0, and the evaluation returns prob_sum /
+    (number_stars - bad_jacobians) = 0 / 0 = NaN, as cpu_evaluate does
+    (evaluation_optimized.c:664-717, :756-784).  A context over the same
+    list gives the integrals and a NaN likelihood."""
+    p, stars = synth.workload(5, synth.WedgeShape(25, 30, 12, 20, 200), 1)
+    none = np.zeros((0, 3))
+    vec = p.parameter_vector()
+    ph, sh, es = holders(p, none)
+    got = sep.evaluate(ph, es, sh, vec)
+    assert math.isnan(got)
+    assert es.es.prob_sum == 0.0 and es.es.num_zero == 0 and es.es.current_star_point == 0
+    ref = O.integrals(*holders(p, stars)[:2])
+    assert rel(es.es.background_integral, ref.es.background_integral) <= INT_TOL
+    assert rel(es.stream_integrals(), ref.stream_integrals()) <= INT_TOL
+    with sep.Context(ParamsHolder(p), StarsHolder(none)) as ctx:
+        for prec in (sep.PREC_FP64, sep.PREC_MIXED):
+            r = ctx.eval(vec, precision=prec)[0]
+            assert math.isnan(r.likelihood)
+            assert rel(r.background_integral, ref.es.background_integral) <= (INT_TOL if prec == sep.PREC_FP64 else 1e-8)
