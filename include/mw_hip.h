@@ -118,6 +118,12 @@ int mw_hip_set_precision(int precision);
  * Context API (no reference counterpart; the reference's nearest equivalent
  * is the per-rank integral_f / likelihood_f plugin pair of
  * milkyway_separation/cpu/astronomy_worker.c:35-124)
+ *
+ * Threads: the calls on one context are not reentrant (each evaluation
+ * reuses the context's staging and result buffers), so a caller that shares
+ * a context between threads serialises its calls; separate contexts may be
+ * used from separate threads.  The drop-in entry points serialise
+ * themselves on one process-wide lock around their cached context.
  * ------------------------------------------------------------------------ */
 
 typedef struct mw_ctx mw_ctx;
