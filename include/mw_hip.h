@@ -344,6 +344,11 @@ const char* mw_last_error(void);
 /* Number of visible GPUs (0 when none). */
 int mw_device_count(void);
 
+/* Makes GPU `device` (0 .. mw_device_count() - 1) the calling thread's
+ * current device: contexts, batches and the drop-in calls made on this thread
+ * afterwards run there (one process per GPU picks its GPU with it). */
+int mw_set_device(int device);
+
 /* Waits for all work of the calling thread's current device. */
 int mw_device_synchronize(void);
 

@@ -535,6 +535,15 @@ int mw_device_count(void)
     return n;
 }
 
+int mw_set_device(int device)
+{
+    const int n = mw_device_count();
+    if (n <= 0) return fail(MW_E_NODEVICE, "no GPU visible");
+    if (device < 0 || device >= n) return fail(MW_E_INVALID, "device index out of range");
+    MW_HIP_CHECK(hipSetDevice(device));
+    return MW_OK;
+}
+
 int mw_device_synchronize(void)
 {
     MW_HIP_CHECK(hipDeviceSynchronize());

@@ -37,7 +37,7 @@ EXPORTS = [
     "mw_hip_evaluate", "mw_hip_calculate_integrals", "mw_hip_calculate_likelihood",
     "mw_hip_set_precision", "mw_ctx_create", "mw_ctx_parameter_count", "mw_ctx_eval",
     "mw_ctx_region_integrals", "mw_ctx_last_timing", "mw_ctx_resident_bytes",
-    "mw_ctx_destroy", "mw_last_error", "mw_device_count", "mw_device_synchronize",
+    "mw_ctx_destroy", "mw_last_error", "mw_device_count", "mw_set_device", "mw_device_synchronize",
     "mw_batch_create", "mw_batch_run", "mw_batch_set_variant", "mw_batch_variant_count",
     "mw_batch_variant_scratch",
     "mw_batch_vector_count", "mw_batch_last_timing", "mw_batch_destroy",
@@ -147,6 +147,8 @@ def load_library() -> C.CDLL:
     L.mw_last_error.restype = C.c_char_p
     L.mw_device_count.argtypes = []
     L.mw_device_count.restype = C.c_int
+    L.mw_set_device.argtypes = [C.c_int]
+    L.mw_set_device.restype = C.c_int
     L.mw_device_synchronize.argtypes = []
     L.mw_device_synchronize.restype = C.c_int
     L.mw_batch_create.argtypes = [C.POINTER(C.c_void_p), C.POINTER(C.c_int), C.c_int, C.c_int,
@@ -256,6 +258,10 @@ def _check(rc: int):
 
 def device_count() -> int:
     return load_library().mw_device_count()
+
+
+def set_device(device: int):
+    _check(load_library().mw_set_device(device))
 
 
 def require_device():
