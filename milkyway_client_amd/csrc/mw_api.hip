@@ -443,13 +443,6 @@ int run_plan(mw_ctx* c, Plan& P)
     return MW_OK;
 }
 
-int make_const(const ASTRONOMY_PARAMETERS& ap, EvalConst* k)
-{
-    std::string err;
-    if (!make_eval_const(ap, k, &err)) return fail(MW_E_INVALID, err);
-    return MW_OK;
-}
-
 // The constant slots of one model (any number of streams) appended to K.
 int make_consts(const ASTRONOMY_PARAMETERS& ap, std::vector<EvalConst>* K)
 {
