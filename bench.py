@@ -332,12 +332,28 @@ def full_size_fp64(steps: int, warmup: int, precision=sep.PREC_FP64):
     med = float(np.median(lat))
     k_ms = float(np.median(kern))
     flops = float(np.mean(points)) * workmodel.flops_per_point(synth.FULL_SIZE_STREAMS)
+    # the star pass beside it: VALU-bound by the frozen model, its HBM rate
+    # from the rocprofv3 traffic of the same launch where it was measured
+    s_ms = float(np.median(star))
+    s_flops = workmodel.star_flops(len(stars), p.convolve, synth.FULL_SIZE_STREAMS)
+    s_bytes = workmodel.star_bytes(len(stars))
+    s_traffic = measured_traffic("star_kernel_fp64_full") if precision == sep.PREC_FP64 else None
+    star_pass = {
+        "bound": "valu", "kernel": "star_kernel (FP64, one work-item per star)",
+        "achieved": round(s_flops / (s_ms * 1e-3) / 1e12, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "frac": round(s_flops / (s_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
+        "algorithmic_bytes": s_bytes, "traffic": s_traffic,
+        "hbm_GBps": round((s_traffic or s_bytes) / (s_ms * 1e-3) / 1e9, 1),
+        "hbm_GBps_of": "measured traffic" if s_traffic else "algorithmic bytes",
+        "hardware": measured_counters("star_kernel_fp64_full") if precision == sep.PREC_FP64 else None,
+    }
     return p, stars, {
         "value": 1.0 / med, "unit": "evals/s", "ms_per_eval_median": 1e3 * med,
         "ms_per_eval_p99": 1e3 * float(np.percentile(lat, 99)),
         "warmup": warmup, "repetitions": steps,
         "integral_kernel_ms_median": round(k_ms, 3),
-        "star_kernel_ms_median": round(float(np.median(star)), 4),
+        "star_kernel_ms_median": round(s_ms, 4),
+        "star_pass": star_pass,
         "achieved_tflops": round(flops / (k_ms * 1e-3) / 1e12, 3),
         "frac_fp64_peak": round(flops / (k_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
         "integrand_evaluations": int(np.mean(points)), "flop_eq": flops,
@@ -724,6 +740,7 @@ def bench_full(args):
         "config": {"workload": "full: " + full["workload"], "precision": args.precision,
                    "batch": 1, "parallelism": "single GPU"},
         "star_kernel_ms": full["star_kernel_ms_median"],
+        "star_pass": full["star_pass"],
         "resident_setup_s": full["resident_setup_s"],
         "roofline": roof,
     }
