@@ -96,3 +96,47 @@ def test_mixed_group_sums_other_stream_counts(n_conv, streams):
     print(f"n = {n_conv}, S = {streams}:", {k: f"{v:.2e}" for k, v in worst.items()})
     assert worst["lnP"] <= LNP_TOL
     assert worst["I_bg"] <= INT_REL_TOL and worst["I_st"] <= INT_REL_TOL
+
+
+def _aux_79():
+    return fixture("79", n_stars=1000, aux=True, convolve=30)
+
+
+def _general_alpha_delta():
+    p, stars = synth.workload(3, synth.WedgeShape(20, 24, 16, 15, 500), 2)
+    p.background_parameters[0] = 1.3
+    p.background_parameters[3] = 0.8
+    return p, stars
+
+
+def _zero_q():
+    p, stars = synth.workload(3, synth.WedgeShape(20, 24, 16, 15, 500), 2)
+    p.background_parameters[1] = 0.0
+    return p, stars
+
+
+@pytest.mark.parametrize("model", [_aux_79, _general_alpha_delta, _zero_q],
+                         ids=["aux_profile", "general_alpha_delta", "zero_q"])
+def test_mixed_request_on_fp64_backgrounds(model):
+    """The mixed kernels cover the Hernquist background with alpha = delta =
+    1 (the reference files' and the benchmark's); for the aux profile,
+    general alpha and delta, and q = 0 both passes run the FP64 kernels.  A
+    mixed request on those models returns the FP64 result bitwise, and that
+    is the oracle's at the FP64 bars (integrals 1e-12, ln L and every star
+    1e-10)."""
+    p, stars = model()
+    ph, sh = ParamsHolder(p), StarsHolder(stars)
+    vec = p.parameter_vector()
+    with sep.Context(ph, sh) as ctx:
+        f = ctx.eval(vec, precision=sep.PREC_FP64, per_star=True)[0]
+        m = ctx.eval(vec, precision=sep.PREC_MIXED, per_star=True)[0]
+    assert (m.likelihood, m.background_integral, m.num_zero) == (f.likelihood, f.background_integral, f.num_zero)
+    assert np.array_equal(m.stream_integrals, f.stream_integrals)
+    assert np.array_equal(m.log10_p, f.log10_p)
+    ref = O.evaluate_full(ParamsHolder(p), StarsHolder(stars))
+    print(model.__name__, rel(f.background_integral, ref.background_integral),
+          rel(f.stream_integrals, ref.stream_integrals), abs(f.likelihood - ref.likelihood))
+    assert rel(f.background_integral, ref.background_integral) <= 1e-12
+    assert rel(f.stream_integrals, ref.stream_integrals) <= 1e-12
+    assert abs(f.likelihood - ref.likelihood) <= 1e-10
+    assert np.max(np.abs(f.log10_p - ref.log10_p)) <= 1e-10
